@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain granular PBD step on one MI355X.
+
+Workload: 1,000,000 particles (r = 0.01) on a 100^3 jittered lattice (seed 1)
+in a 2.4^3 m box (60^3 cells of L = 0.0404, 84,968 wall particles), dt =
+0.002 s and g = -9.8 (the reference scenes' values, scenes/*.json), 3
+stabilisation + 10 solver sweeps, friction 0.5/0.4, FP32.  (With dt = 1/240
+the reference's constraint averaging cannot hold a 100-layer column: most
+particles are pushed through the floor lattice and leave the grid.)  A step = hash + stable sort + cell table + permute/predict +
+neighbour lists + 13 contact sweeps + finalize, inputs resident in HBM.
+
+Prints one JSON line.  `value` = particle-steps/s over the timed steps.
+`roofline` covers the dominant kernel (the contact sweep), timed live with
+hipEvents on its stream; its algorithmic bytes per launch are
+N*32 + N_b*12 (read x*, m, m_s and write x* per particle; boundary positions
+once).  `cpu_baseline` runs the C++/OpenMP FP32 oracle on a bounded sample
+of the same state.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s at 1M particles + % HBM roofline; upsampled fine-particles/s"
+
+
+def bytes_per_particle_step(n, cells, nb, S, K):
+    """Algorithmic bytes of an ideally fused step (compulsory traffic)."""
+    return 208 + 8 * cells / n + K * (32 + 12 * nb / n) + 24 * S
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M particles)")
+    ap.add_argument("--box", type=float, default=2.4)
+    ap.add_argument("--dt", type=float, default=0.002)
+    ap.add_argument("--cpu-baseline-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stage-timing", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU slab "
+                          "decomposition is a design (DESIGN.md), not built",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from granular_gpu_amd import _lib, scenes
+    from granular_gpu_amd.device import PlainSim
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}), flush=True)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    lib = _lib.load()
+    if lib.gg_device_check() != 0:
+        print(json.dumps({"error": "device is not gfx950"}), flush=True)
+        sys.exit(1)
+
+    S, I, mu_s, mu_k = 3, 10, 0.5, 0.4
+    scene = scenes.plain_box(args.side, args.box, dt=args.dt, g=-9.8)
+    n, nb = scene.n, scene.boundary.shape[0]
+    cells = scene.cells[0] * scene.cells[1] * scene.cells[2]
+    sim = PlainSim(scene, stab=S, solver=I, mu_s=mu_s, mu_k=mu_k)
+    print(f"# scene {scene.name}: n={n} walls={nb} cells={cells}", file=sys.stderr, flush=True)
+
+    sim.step(args.warmup)
+    torch.cuda.synchronize()
+    print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+
+    K = S + I
+    if not args.no_stage_timing:
+        lib.gg_profiler_enable(args.steps * (K + 4) + 16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    print(f"# timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    stage_ms = {}
+    roofline = None
+    if not args.no_stage_timing:
+        tot = (C.c_double * _lib.STAGES.__len__())()
+        cnt = (C.c_int * len(_lib.STAGES))()
+        _lib.check("gg_profiler_collect", lib.gg_profiler_collect(tot, cnt, len(_lib.STAGES)))
+        lib.gg_profiler_enable(0)
+        for k, name in enumerate(_lib.STAGES):
+            if cnt[k]:
+                stage_ms[name] = {"total_ms": round(tot[k], 4), "launches": cnt[k],
+                                  "avg_us": round(1000.0 * tot[k] / cnt[k], 3)}
+        sw = stage_ms.get("sweep")
+        if sw:
+            per_launch_bytes = n * 32 + nb * 12
+            avg_s = sw["avg_us"] * 1e-6
+            achieved = per_launch_bytes / avg_s / 1e9
+            traffic = None
+            pmc = ROOT / "profiles" / "pmc_sweep_m1.json"
+            if pmc.exists():
+                try:
+                    traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
+                except Exception:
+                    traffic = None
+            roofline = {"kernel": "k_sweep (contact sweep)", "bound": "hbm",
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "algorithmic_bytes_per_launch": per_launch_bytes,
+                        "avg_launch_us": sw["avg_us"]}
+
+    value = n * args.steps / elapsed
+    bps = bytes_per_particle_step(n, cells, nb, S, K)
+    step_frac = value * bps / (HBM_PEAK_GBS * 1e9)
+
+    pos, vel, mass = sim.host()
+    assert np.isfinite(pos).all() and np.isfinite(vel).all(), "non-finite state"
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import pyoracle as po
+        olib = po.load()
+        p = po.params_from(scene, S, I, mu_s, mu_k)
+        bpos, bcs = po.sorted_boundary(scene.boundary, scene.cell_length, scene.cells)
+        steps_done = 0
+        cp, cv, cm = pos, vel, mass
+        tc0 = time.perf_counter()
+        while steps_done < 8:
+            cp, cv, cm = po.plain_steps_f32(cp, cv, cm, bpos, bcs, p, 1)[:3]
+            steps_done += 1
+            if time.perf_counter() - tc0 >= args.cpu_baseline_seconds:
+                break
+        tc = time.perf_counter() - tc0
+        cpu = {"value": round(n * steps_done / tc, 1), "unit": "particle-steps/s",
+               "cores": olib.oracle_get_threads(), "kind": "port",
+               "sample": f"{steps_done} step(s) of the same 1M-particle state after the GPU "
+                         f"warmup+timed steps, FP32 oracle (C++/OpenMP, full 27-cell scans)"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain box {args.side}^3 jittered lattice (r=0.01, seed 1) in a "
+                        f"{args.box}^3 m box, {scene.cells[0]}^3 cells, {nb} wall particles, "
+                        f"dt={args.dt}, g=-9.8, {S} stabilisation + {I} solver sweeps, "
+                        f"friction {mu_s}/{mu_k}, FP32, steps {args.warmup}.."
+                        f"{args.warmup + args.steps}",
+            "particles": n, "boundary_particles": nb, "cells": cells,
+            "parallelism": "single GPU",
+        },
+        "step_hbm_model": {"bytes_per_particle_step": round(bps, 1),
+                           "frac_of_hbm_roofline": round(step_frac, 4)},
+        "stages": stage_ms,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

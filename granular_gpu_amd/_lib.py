@@ -1,0 +1,132 @@
+"""ctypes binding of libgranular_gpu.so (the C-ABI in include/granular_gpu.h).
+
+The library is the product: every compute call goes to its HIP kernels.  If
+the shared object is missing this module raises immediately; there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libgranular_gpu.so"
+
+GG_OK = 0
+GG_ERR_INVALID = -1
+GG_ERR_CAPACITY = -2
+GG_ERR_HIP = -3
+GG_ERR_WORKSPACE = -4
+GG_ERR_NO_DEVICE = -5
+GG_FLAG_NO_NLIST = 1
+STAGES = ["sort", "predict", "nlist", "sweep", "finalize"]
+
+
+class gg_grid(C.Structure):
+    _fields_ = [("cell_length", C.c_float), ("cx", C.c_int), ("cy", C.c_int), ("cz", C.c_int)]
+
+
+class gg_params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float), ("gx", C.c_float), ("gy", C.c_float), ("gz", C.c_float),
+        ("density", C.c_float), ("stab_iters", C.c_int), ("solver_iters", C.c_int),
+        ("mu_s", C.c_float), ("mu_k", C.c_float), ("boundary_radius", C.c_float),
+        ("contact_radius_min", C.c_float), ("sleep_speed", C.c_float),
+        ("damp_speed", C.c_float), ("damping", C.c_float), ("adaptive_damping", C.c_float),
+        ("skin", C.c_float), ("flags", C.c_int),
+    ]
+
+
+class gg_particles(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("capacity", C.c_int), ("pos", C.c_void_p), ("vel", C.c_void_p),
+        ("mass", C.c_void_p), ("scaled_mass", C.c_void_p), ("surface", C.c_void_p),
+        ("surface_distance", C.c_void_p), ("num_surface", C.c_void_p),
+        ("particle_2_cell", C.c_void_p), ("is_animated", C.c_void_p),
+        ("adaptive_last_step", C.c_void_p),
+    ]
+
+
+class gg_adaptive(C.Structure):
+    _fields_ = [("remove", C.c_void_p), ("merge", C.c_void_p), ("merge_velocity", C.c_void_p),
+                ("merge_count", C.c_void_p)]
+
+
+class gg_workspace(C.Structure):
+    _fields_ = [
+        ("base", C.c_void_p), ("bytes", C.c_size_t), ("capacity", C.c_int),
+        ("boundary_capacity", C.c_int), ("grid", gg_grid), ("max_sweeps", C.c_int),
+        ("gmax", C.c_int), ("bmax", C.c_int),
+    ]
+
+
+P = C.POINTER
+VP = C.c_void_p
+
+# name -> (restype, argtypes); mirrors include/granular_gpu.h
+_PROTOS = {
+    "gg_version": (C.c_char_p, []),
+    "gg_status_string": (C.c_char_p, [C.c_int]),
+    "gg_device_check": (C.c_int, []),
+    "gg_params_default": (None, [P(gg_params)]),
+    "gg_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, gg_grid, C.c_int, C.c_int, C.c_int]),
+    "gg_workspace_init": (C.c_int, [P(gg_workspace), VP]),
+    "gg_grid_build": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, VP, P(gg_workspace), VP]),
+    "gg_boundary_build": (C.c_int, [P(gg_particles), VP, P(gg_workspace), VP]),
+    "gg_step": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, P(gg_particles), VP, P(gg_params),
+                          P(gg_workspace), VP]),
+    "gg_predict": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, P(gg_params),
+                             P(gg_workspace), VP]),
+    "gg_contact_sweep": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, P(gg_params), C.c_int,
+                                   P(gg_workspace), VP]),
+    "gg_finalize": (C.c_int, [P(gg_particles), P(gg_params), P(gg_workspace), VP]),
+    "gg_workspace_view": (VP, [P(gg_workspace), C.c_int]),
+    "gg_profiler_enable": (C.c_int, [C.c_int]),
+    "gg_profiler_collect": (C.c_int, [P(C.c_double), P(C.c_int), C.c_int]),
+}
+
+_lib = None
+
+
+def declared_symbols() -> list[str]:
+    """Names of the entry points this binding declares."""
+    return sorted(_PROTOS)
+
+
+def load() -> C.CDLL:
+    """Load the HIP library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} not found: build it with `make -C granular_gpu_amd/csrc` "
+            "(or __graft_entry__.build()); there is no non-HIP fallback")
+    lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_GLOBAL if hasattr(os, "RTLD_GLOBAL") else 0)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class GGError(RuntimeError):
+    def __init__(self, fn: str, status: int):
+        msg = load().gg_status_string(status)
+        super().__init__(f"{fn} failed: {status} ({msg.decode() if msg else '?'})")
+        self.status = status
+
+
+def check(fn: str, status: int) -> None:
+    if status != GG_OK:
+        raise GGError(fn, status)
+
+
+def default_params() -> gg_params:
+    p = gg_params()
+    load().gg_params_default(C.byref(p))
+    return p

@@ -1,0 +1,137 @@
+// Internal helpers of the granular_gpu library: workspace layout, launch
+// checks, shared constants.  Host and device code include this file.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "granular_gpu.h"
+
+namespace gg {
+
+constexpr int kBlock = 256;        // threads per block for particle kernels
+constexpr int kScanTile = 2048;    // elements per block in the cell-table scan
+constexpr int kScanMaxParts = 8192;
+constexpr int kSmallCell = 32;     // cells up to this size are sorted per thread
+constexpr int kBigLds = 4096;      // cells up to this size are sorted in LDS
+constexpr double kPiD = 3.14159265358979323846;
+constexpr float kPiF = 3.14159265358979323846f;
+// A particle's list stays exact while its own accumulated correction and
+// that of every particle of its stencil stay below half of this fraction of
+// the skin.
+constexpr float kListValidFrac = 0.9f;
+constexpr int kFallbackSlots = 62;  // ctr[1 + k % 62] = full-stencil count of sweep k
+
+// Byte offsets of every buffer inside the caller's workspace.  Every region
+// starts on a 256-byte boundary.
+struct Layout {
+  size_t count;    // int[C+1]       per-cell histogram, zero between builds
+  size_t key;      // int[cmax]      cell id per particle (input order)
+  size_t slot;     // int[cmax]      arrival slot inside the cell
+  size_t perm;     // int[cmax]      sorted slot -> old index
+  size_t itmp;     // int[cmax]      big-cell sort scratch
+  size_t part;     // int[parts+1]   scan partials
+  size_t big;      // int[C+1]       list of cells needing the big sort
+  size_t ctr;      // int[64]        device counters ([0] = big-cell count)
+  size_t dirty;    // int[C+1]       cells near a particle past half the skin
+  size_t x0;       // float4[cap]    start-of-step position + radius
+  size_t off0;     // float4[cap]    offset ping
+  size_t off1;     // float4[cap]    offset pong
+  size_t d0;       // float4[cap]    dt*v after gravity
+  size_t xb;       // float4[cap]    position after stabilisation + radius
+  size_t meta;     // int4[cap]      neighbour list meta
+  size_t glist;    // int[gmax*cap]  granular candidates, [k][i]
+  size_t blist;    // int[bmax*cap]  boundary candidates, [k][i]
+  size_t ftmp0;    // float[cmax]    payload staging
+  size_t itmp0;    // int[cmax]
+  size_t itmp1;    // int[cmax]
+  size_t ftmp1;    // float[cmax]
+  size_t v3tmp0;   // float3[cmax]
+  size_t v3tmp1;   // float3[cmax]
+  size_t bnd4;     // float4[bcap]   packed sorted boundary
+  size_t total;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+inline long long grid_cells(const gg_grid &g) {
+  return (long long)g.cx * (long long)g.cy * (long long)g.cz;
+}
+
+inline Layout make_layout(int capacity, int bcap, gg_grid g, int max_sweeps,
+                          int gmax, int bmax) {
+  Layout L{};
+  const size_t C1 = (size_t)grid_cells(g) + 1;
+  const size_t cmax = (size_t)(capacity > bcap ? capacity : bcap) + 1;
+  const size_t cap = (size_t)capacity + 1;
+  const size_t parts = (C1 + kScanTile - 1) / kScanTile + 1;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = o;
+    o = align256(o + bytes);
+    return at;
+  };
+  L.count = take(C1 * 4);
+  L.key = take(cmax * 4);
+  L.slot = take(cmax * 4);
+  L.perm = take(cmax * 4);
+  L.itmp = take(cmax * 4);
+  L.part = take(parts * 4);
+  L.big = take(C1 * 4);
+  L.ctr = take(64 * 4);
+  L.dirty = take(C1 * 4);
+  L.x0 = take(cap * 16);
+  L.off0 = take(cap * 16);
+  L.off1 = take(cap * 16);
+  L.d0 = take(cap * 16);
+  L.xb = take(cap * 16);
+  L.meta = take(cap * 16);
+  L.glist = take((size_t)gmax * cap * 4);
+  L.blist = take((size_t)bmax * cap * 4);
+  L.ftmp0 = take(cmax * 4);
+  L.itmp0 = take(cmax * 4);
+  L.itmp1 = take(cmax * 4);
+  L.ftmp1 = take(cmax * 4);
+  L.v3tmp0 = take(cmax * 12);
+  L.v3tmp1 = take(cmax * 12);
+  L.bnd4 = take(((size_t)bcap + 1) * 16);
+  L.total = o;
+  return L;
+}
+
+template <class T> inline T *at(const gg_workspace *ws, size_t off) {
+  return reinterpret_cast<T *>(static_cast<char *>(ws->base) + off);
+}
+
+inline int check_ws(const gg_workspace *ws) {
+  if (!ws || !ws->base) return GG_ERR_INVALID;
+  if (ws->capacity < 0 || ws->boundary_capacity < 0 || ws->max_sweeps < 0 ||
+      ws->gmax < 0 || ws->bmax < 0)
+    return GG_ERR_INVALID;
+  if (ws->grid.cx <= 0 || ws->grid.cy <= 0 || ws->grid.cz <= 0 ||
+      !(ws->grid.cell_length > 0.0f))
+    return GG_ERR_INVALID;
+  const long long C1 = grid_cells(ws->grid) + 1;
+  if (C1 > (long long)kScanTile * kScanMaxParts) return GG_ERR_INVALID;
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid,
+                         ws->max_sweeps, ws->gmax, ws->bmax);
+  if (ws->bytes < L.total) return GG_ERR_WORKSPACE;
+  return GG_OK;
+}
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? GG_OK : GG_ERR_HIP; }
+
+}  // namespace gg
+
+#define GG_LAUNCH_CHECK()                                  \
+  do {                                                     \
+    hipError_t _e = hipGetLastError();                     \
+    if (_e != hipSuccess) return GG_ERR_HIP;               \
+  } while (0)
+
+#define GG_TRY(expr)                                       \
+  do {                                                     \
+    int _s = (expr);                                       \
+    if (_s != GG_OK) return _s;                            \
+  } while (0)

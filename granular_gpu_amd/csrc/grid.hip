@@ -1,0 +1,409 @@
+// Uniform-grid neighbour search: cell hash, stable counting sort by cell id,
+// cell table (cellStart) and permutation of the particle payload.
+//
+// Reference: GranularSystem.cu:65-280 (neighbor_search_granular/boundary),
+// CUDAFunctions.cuh:54-107 (mapParticles2Cells_CUDA, countingInCell_CUDA),
+// thrust::sort_by_key / gather / exclusive_scan.  The sort here is a
+// counting sort: integer atomics give each particle a slot in its cell, the
+// cell table is the exclusive scan of the histogram, and a per-cell sort by
+// old index then restores stability, so the permutation equals a stable
+// sort by cell id (what thrust's radix sort_by_key returns) on every run.
+#include <hip/hip_runtime.h>
+
+#include "gg_device.hpp"
+#include "gg_internal.hpp"
+#include "grid_kernels.hpp"
+
+namespace gg {
+
+// key_i = hash(x_i); slot_i = arrival order inside the cell.
+__global__ void __launch_bounds__(kBlock) k_hash_count(const float *__restrict__ pos, int n,
+                                                       DevGrid g, int *__restrict__ key,
+                                                       int *__restrict__ slot,
+                                                       int *__restrict__ count,
+                                                       int *__restrict__ ctr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) ctr[0] = 0;  // big-cell counter of this build
+  // All lanes take part in the wave operations; lanes past n carry key -1.
+  int c = -1;
+  if (i < n) {
+    const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+    c = hash_cell(px, py, pz, g);
+    key[i] = c;
+  }
+  // Wave-aggregated slot reservation: each run of equal keys in adjacent
+  // lanes (the input is the previous step's cell order, so runs are long)
+  // reserves its slots with one atomic; lanes take consecutive slots.
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(c, 1, 64);
+  const bool head = (lane == 0) || (prev != c);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = heads & ((2ull << lane) - 1ull);  // heads at lanes <= lane
+  const int hl = 63 - __clzll((long long)upto);                       // my run's head lane
+  const unsigned long long after = heads & ~((2ull << lane) - 1ull);  // heads above lane
+  const int nl = after ? __ffsll((long long)after) - 1 : 64;          // next run's head lane
+  int basev = 0;
+  if (head && c >= 0) {
+    // run length = lanes up to the next head (lanes past n are key -1 runs)
+    basev = atomicAdd(&count[c], nl - lane);
+  }
+  basev = __shfl(basev, hl, 64);
+  if (i < n) slot[i] = basev + (lane - hl);
+}
+
+// Block sums of kScanTile-element tiles of count[0..m).
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const int *__restrict__ count, int m,
+                                                        int *__restrict__ part) {
+  __shared__ int wsum[kBlock / 64];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * 8;
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int idx = base + k;
+    if (idx < m) s += count[idx];
+  }
+  s = wave_sum(s);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wsum[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += wsum[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of the (<= kScanMaxParts) partials in one block.
+__global__ void __launch_bounds__(1024) k_scan_parts(int *__restrict__ part, int np) {
+  __shared__ int buf[kScanMaxParts];
+  __shared__ int wtot[16];
+  const int t = threadIdx.x;
+  for (int k = t; k < np; k += 1024) buf[k] = part[k];
+  __syncthreads();
+  // each thread owns 8 consecutive entries
+  const int b0 = t * 8;
+  int loc[8];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    loc[k] = (b0 + k < np) ? buf[b0 + k] : 0;
+    s += loc[k];
+  }
+  // inclusive wave scan of per-thread sums
+  const int lane = t & 63, wid = t >> 6;
+  int inc = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 63) wtot[wid] = inc;
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int w = 0; w < 16; ++w) {
+      const int v = wtot[w];
+      wtot[w] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = wtot[wid] + inc - s;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (b0 + k < np) buf[b0 + k] = run;
+    run += loc[k];
+  }
+  __syncthreads();
+  for (int k = t; k < np; k += 1024) part[k] = buf[k];
+}
+
+// cell_start[c] = exclusive prefix of count; cell_start[m] = total.  Clears
+// count for the next build.
+__global__ void __launch_bounds__(kBlock) k_scan_apply(int *__restrict__ count, int m,
+                                                       const int *__restrict__ part,
+                                                       int *__restrict__ cell_start) {
+  __shared__ int wtot[kBlock / 64];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * 8;
+  int loc[8];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int idx = base + k;
+    loc[k] = idx < m ? count[idx] : 0;
+    s += loc[k];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int inc = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 63) wtot[wid] = inc;
+  __syncthreads();
+  int woff = 0;
+  for (int w = 0; w < wid; ++w) woff += wtot[w];
+  int run = part[blockIdx.x] + woff + inc - s;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int idx = base + k;
+    if (idx < m) {
+      cell_start[idx] = run;
+      count[idx] = 0;
+      if (idx == m - 1) cell_start[m] = run + loc[k];
+    }
+    run += loc[k];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter(const int *__restrict__ key,
+                                                    const int *__restrict__ slot, int n,
+                                                    const int *__restrict__ cell_start,
+                                                    int *__restrict__ perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int d = cell_start[key[i]] + slot[i];
+  if (d >= 0 && d < n) perm[d] = i;
+}
+
+// One thread per cell: insertion sort of small cells by old index; larger
+// cells are queued for k_sort_big.
+__global__ void __launch_bounds__(kBlock) k_sort_small(const int *__restrict__ cell_start, int m,
+                                                       int *__restrict__ perm,
+                                                       int *__restrict__ big,
+                                                       int *__restrict__ ctr) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  const int s = cell_start[c], e = cell_start[c + 1];
+  const int len = e - s;
+  if (len <= 1) return;
+  if (len > kSmallCell) {
+    big[atomicAdd(&ctr[0], 1)] = c;
+    return;
+  }
+  for (int a = s + 1; a < e; ++a) {
+    const int v = perm[a];
+    int b = a - 1;
+    while (b >= s && perm[b] > v) {
+      perm[b + 1] = perm[b];
+      --b;
+    }
+    perm[b + 1] = v;
+  }
+}
+
+// One block per queued cell (grid-stride over the queue).  Cells up to
+// kBigLds entries: bitonic sort of the old indices in LDS.  Larger cells
+// (e.g. the sentinel cell of particles outside the grid): stable
+// compaction of {i : key[i] == c} over the whole key array in index order,
+// one 1024-key tile at a time (ballot + block prefix sum), which is O(n)
+// and yields ascending old indices directly.
+__global__ void __launch_bounds__(kBlock) k_sort_big(const int *__restrict__ cell_start,
+                                                     int *__restrict__ perm,
+                                                     const int *__restrict__ big,
+                                                     const int *__restrict__ ctr,
+                                                     const int *__restrict__ key, int n) {
+  __shared__ int buf[kBigLds];
+  __shared__ int wcnt[kBlock / 64];
+  const int nbig = ctr[0];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+    const int c = big[b];
+    const int s = cell_start[c], e = cell_start[c + 1];
+    const int len = e - s;
+    if (len <= kBigLds) {
+      int P = 1;
+      while (P < len) P <<= 1;
+      for (int k = threadIdx.x; k < P; k += blockDim.x) buf[k] = k < len ? perm[s + k] : 0x7fffffff;
+      __syncthreads();
+      for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int t = threadIdx.x; t < P; t += blockDim.x) {
+            const int u = t ^ j;
+            if (u > t) {
+              const int x = buf[t], y = buf[u];
+              const bool up = (t & k) == 0;
+              if ((x > y) == up) {
+                buf[t] = y;
+                buf[u] = x;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      for (int k = threadIdx.x; k < len; k += blockDim.x) perm[s + k] = buf[k];
+      __syncthreads();
+    } else {
+      int base = 0;
+      for (int t0 = 0; t0 < n; t0 += 4 * kBlock) {
+        // thread owns keys t0 + 4*tid .. +3 (index order within the tile)
+        const int i0 = t0 + 4 * threadIdx.x;
+        int m[4];
+        int cnt = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          m[q] = (i0 + q < n && key[i0 + q] == c) ? 1 : 0;
+          cnt += m[q];
+        }
+        int inc = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+          const int v = __shfl_up(inc, o, 64);
+          if (lane >= o) inc += v;
+        }
+        if (lane == 63) wcnt[wid] = inc;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+          if (w < wid) woff += wcnt[w];
+          tot += wcnt[w];
+        }
+        int at = base + woff + inc - cnt;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (m[q]) {
+            if (at < len) perm[s + at] = i0 + q;
+            ++at;
+          }
+        }
+        base += tot;
+        __syncthreads();
+      }
+    }
+  }
+}
+
+// dst[k] = src[perm[k]] for records of `words` 32-bit words.
+__global__ void __launch_bounds__(kBlock) k_gather_words(const uint32_t *__restrict__ src,
+                                                         uint32_t *__restrict__ dst,
+                                                         const int *__restrict__ perm, int n,
+                                                         int words) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int o = perm[k];
+  for (int w = 0; w < words; ++w) dst[(size_t)k * words + w] = src[(size_t)o * words + w];
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather_keys(const int *__restrict__ key,
+                                                        const int *__restrict__ perm, int n,
+                                                        int *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  out[k] = key[perm[k]];
+}
+
+__global__ void __launch_bounds__(kBlock) k_pack_boundary(const float *__restrict__ pos, int n,
+                                                          float rb, float4 *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  out[k] = make_float4(pos[3 * k], pos[3 * k + 1], pos[3 * k + 2], rb);
+}
+
+static inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// Hash, counting sort, cell table and perm for `n` positions.
+int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *ws,
+                 hipStream_t st) {
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  const DevGrid g = dev_grid(ws->grid);
+  const int m = (int)(grid_cells(ws->grid) + 1);  // C+1 bins incl. sentinel
+  int *count = at<int>(ws, L.count);
+  int *key = at<int>(ws, L.key);
+  int *slot = at<int>(ws, L.slot);
+  int *perm = at<int>(ws, L.perm);
+  int *part = at<int>(ws, L.part);
+  int *big = at<int>(ws, L.big);
+  int *ctr = at<int>(ws, L.ctr);
+  const int nb = n > 0 ? blocks_for(n) : 1;
+  k_hash_count<<<nb, kBlock, 0, st>>>(pos, n, g, key, slot, count, ctr);
+  GG_LAUNCH_CHECK();
+  const int np = (m + kScanTile - 1) / kScanTile;
+  k_scan_reduce<<<np, kBlock, 0, st>>>(count, m, part);
+  GG_LAUNCH_CHECK();
+  k_scan_parts<<<1, 1024, 0, st>>>(part, np);
+  GG_LAUNCH_CHECK();
+  k_scan_apply<<<np, kBlock, 0, st>>>(count, m, part, cell_start);
+  GG_LAUNCH_CHECK();
+  if (n > 0) {
+    k_scatter<<<blocks_for(n), kBlock, 0, st>>>(key, slot, n, cell_start, perm);
+    GG_LAUNCH_CHECK();
+  }
+  k_sort_small<<<blocks_for(m), kBlock, 0, st>>>(cell_start, m, perm, big, ctr);
+  GG_LAUNCH_CHECK();
+  k_sort_big<<<256, kBlock, 0, st>>>(cell_start, perm, big, ctr, key, n);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+
+static int gather_inplace(void *arr, int words, const int *perm, int n, void *tmp,
+                          hipStream_t st) {
+  if (!arr || n <= 0) return GG_OK;
+  k_gather_words<<<blocks_for(n), kBlock, 0, st>>>(static_cast<const uint32_t *>(arr),
+                                                   static_cast<uint32_t *>(tmp), perm, n, words);
+  GG_LAUNCH_CHECK();
+  return hip_status(
+      hipMemcpyAsync(arr, tmp, (size_t)n * words * 4, hipMemcpyDeviceToDevice, st));
+}
+
+int grid_build_general(gg_particles *set, gg_adaptive *ad, int *cell_start, int *perm_out,
+                       const gg_workspace *ws, hipStream_t st, int capacity_limit) {
+  if (!set || !cell_start) return GG_ERR_INVALID;
+  GG_TRY(check_ws(ws));
+  const int n = set->n;
+  if (n < 0) return GG_ERR_INVALID;
+  if (n > capacity_limit) return GG_ERR_CAPACITY;
+  if (n > 0 && !set->pos) return GG_ERR_INVALID;
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  GG_TRY(sort_by_cell(reinterpret_cast<const float *>(set->pos), n, cell_start, ws, st));
+  const int *perm = at<int>(ws, L.perm);
+  const int *key = at<int>(ws, L.key);
+  if (n == 0) return GG_OK;
+  void *t3 = at<void>(ws, L.v3tmp0);
+  void *t1 = at<void>(ws, L.itmp0);
+  GG_TRY(gather_inplace(set->pos, 3, perm, n, t3, st));
+  GG_TRY(gather_inplace(set->vel, 3, perm, n, t3, st));
+  GG_TRY(gather_inplace(set->mass, 1, perm, n, t1, st));
+  GG_TRY(gather_inplace(set->is_animated, 1, perm, n, t1, st));
+  if (ad) {
+    GG_TRY(gather_inplace(ad->merge, 1, perm, n, t1, st));
+    GG_TRY(gather_inplace(ad->merge_count, 1, perm, n, t1, st));
+    GG_TRY(gather_inplace(ad->remove, 1, perm, n, t1, st));
+  }
+  if (set->particle_2_cell) {
+    k_gather_keys<<<blocks_for(n), kBlock, 0, st>>>(key, perm, n, set->particle_2_cell);
+    GG_LAUNCH_CHECK();
+  }
+  if (perm_out)
+    GG_TRY(hip_status(
+        hipMemcpyAsync(perm_out, perm, (size_t)n * 4, hipMemcpyDeviceToDevice, st)));
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_grid_build(gg_particles *set, gg_adaptive *ad, int *cell_start, int *perm_out,
+                             const gg_workspace *ws, hipStream_t stream) {
+  if (!ws) return GG_ERR_INVALID;
+  return grid_build_general(set, ad, cell_start, perm_out, ws, stream, ws->capacity);
+}
+
+extern "C" int gg_boundary_build(gg_particles *boundary, int *bcell_start,
+                                 const gg_workspace *ws, hipStream_t stream) {
+  if (!boundary || !bcell_start || !ws) return GG_ERR_INVALID;
+  GG_TRY(grid_build_general(boundary, nullptr, bcell_start, nullptr, ws, stream,
+                            ws->boundary_capacity));
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  if (boundary->n > 0) {
+    gg_params dp;
+    gg_params_default(&dp);
+    k_pack_boundary<<<blocks_for(boundary->n), kBlock, 0, stream>>>(
+        reinterpret_cast<const float *>(boundary->pos), boundary->n, dp.boundary_radius,
+        at<float4>(ws, L.bnd4));
+    GG_LAUNCH_CHECK();
+  }
+  return GG_OK;
+}

@@ -1,0 +1,81 @@
+// Stage profiler: hipEvents bracketing each stage on the stream it runs on.
+#include <hip/hip_runtime.h>
+
+#include <utility>
+#include <vector>
+
+#include "gg_internal.hpp"
+#include "profiler.hpp"
+
+namespace gg {
+namespace {
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<std::pair<int, int>> rec;  // (stage, first event index)
+  int next = 0;
+  int open_stage = -1;
+};
+Prof &prof() {
+  static Prof p;
+  return p;
+}
+}  // namespace
+
+void prof_begin(int stage, hipStream_t st) {
+  Prof &p = prof();
+  if (!p.on || p.next + 2 > (int)p.ev.size()) {
+    p.open_stage = -1;
+    return;
+  }
+  if (hipEventRecord(p.ev[p.next], st) != hipSuccess) {
+    p.open_stage = -1;
+    return;
+  }
+  p.open_stage = stage;
+}
+
+void prof_end(int stage, hipStream_t st) {
+  Prof &p = prof();
+  if (!p.on || p.open_stage != stage) return;
+  if (hipEventRecord(p.ev[p.next + 1], st) != hipSuccess) return;
+  p.rec.emplace_back(stage, p.next);
+  p.next += 2;
+  p.open_stage = -1;
+}
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_profiler_enable(int max_records) {
+  Prof &p = prof();
+  for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
+  p.ev.clear();
+  p.rec.clear();
+  p.next = 0;
+  p.on = false;
+  if (max_records <= 0) return GG_OK;
+  p.ev.resize(2 * (size_t)max_records);
+  for (auto &e : p.ev)
+    if (hipEventCreate(&e) != hipSuccess) return GG_ERR_HIP;
+  p.on = true;
+  return GG_OK;
+}
+
+extern "C" int gg_profiler_collect(double *total_ms, int *count, int nstages) {
+  Prof &p = prof();
+  if (!total_ms || !count || nstages < 0) return GG_ERR_INVALID;
+  for (auto &r : p.rec) {
+    if (hipEventSynchronize(p.ev[r.second + 1]) != hipSuccess) return GG_ERR_HIP;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.ev[r.second], p.ev[r.second + 1]) != hipSuccess)
+      return GG_ERR_HIP;
+    if (r.first >= 0 && r.first < nstages) {
+      total_ms[r.first] += ms;
+      count[r.first] += 1;
+    }
+  }
+  p.rec.clear();
+  p.next = 0;
+  return GG_OK;
+}

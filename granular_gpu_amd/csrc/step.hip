@@ -1,0 +1,543 @@
+// Plain PBD step: permute + predict, neighbour candidate lists, contact
+// sweeps (stabilisation + solver), finalize.
+//
+// Reference: Solver::step (Solver.cu:82-109) -> add_external_force,
+// update_particle_positions, apply_mass_scaling (Solver.cu:125-191),
+// project (Solver.cu:416-513) with compute_delta_pos / boundary_constraint /
+// particles_constraint (Solver.cu:272-401), final_update (Solver.cu:193-270).
+//
+// Deterministic semantics (the reference scatters into delta_pos[j] and
+// n[j] without atomics): each sweep is a gather.  For particle i,
+//   boundary k in contact (d < R_b + max(r_i, rmin)): A += ((rc-d)/d) p, n += 1
+//   granular j in contact (d <= r_i + r_j)         : A += ((rc-d)/d) p, n += 2
+// which is the reference's own term plus the mirror term thread j scatters
+// into i (their sum is independent of the mass scaling).  Then
+// x* += A / max(n, 1).  Positions are carried as base + offset: the offset
+// D = x* - x stays small, so differences and the final x = base + D are
+// rounded once.
+//
+// Neighbour candidate lists: built once per step with a skin; a sweep uses
+// particle i's list only when (a) the stencil base of x*_i is the one the
+// list was built for and (b) i's accumulated displacement plus the largest
+// displacement of any particle is below 0.9*skin, which guarantees every
+// pair outside the list is still out of contact.  Otherwise the thread scans
+// the full 27-cell stencil.  Both paths visit contacts in the same order,
+// so the result is identical to always scanning the stencil.
+#include <hip/hip_runtime.h>
+
+#include "gg_device.hpp"
+#include "gg_internal.hpp"
+#include "grid_kernels.hpp"
+#include "profiler.hpp"
+
+namespace gg {
+
+__global__ void __launch_bounds__(kBlock)
+    k_permute_predict(int n, const int *__restrict__ perm, const int *__restrict__ key,
+                      const float *__restrict__ pos, const float *__restrict__ vel,
+                      const float *__restrict__ mass, float4 *__restrict__ x0,
+                      float4 *__restrict__ off0, float4 *__restrict__ d0,
+                      float *__restrict__ mass_tmp, float *__restrict__ ms_out,
+                      int *__restrict__ p2c, const int *__restrict__ rem_in,
+                      const float *__restrict__ mer_in, const int *__restrict__ mc_in,
+                      int *__restrict__ rem_tmp, float *__restrict__ mer_tmp,
+                      int *__restrict__ mc_tmp, float dt, float dvx, float dvy, float dvz,
+                      double denom, int *__restrict__ dirty, int ndirty,
+                      int *__restrict__ fbcount) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = i; k < ndirty; k += gridDim.x * blockDim.x) dirty[k] = 0;
+  if (i < kFallbackSlots) fbcount[i] = 0;
+  if (i >= n) return;
+  const int o = perm ? perm[i] : i;
+  const float px = pos[3 * o], py = pos[3 * o + 1], pz = pos[3 * o + 2];
+  float vx = vel[3 * o], vy = vel[3 * o + 1], vz = vel[3 * o + 2];
+  vx = vx + dvx;
+  vy = vy + dvy;
+  vz = vz + dvz;
+  const float Dx = dt * vx, Dy = dt * vy, Dz = dt * vz;
+  const float m = mass[o];
+  const float r = radius_contact(m, denom);
+  x0[i] = make_float4(px, py, pz, r);
+  off0[i] = make_float4(Dx, Dy, Dz, 0.0f);
+  d0[i] = make_float4(Dx, Dy, Dz, 0.0f);
+  mass_tmp[i] = m;
+  if (ms_out) ms_out[i] = m * exp_f(-py);
+  if (perm && p2c) p2c[i] = key[o];
+  if (rem_in) rem_tmp[i] = rem_in[o];
+  if (mer_in) mer_tmp[i] = mer_in[o];
+  if (mc_in) mc_tmp[i] = mc_in[o];
+}
+
+// Candidate lists: particles of the stencil of trunc(x*/L) with
+// d < rc + skin, in scan order (cells ascending, index ascending).
+__global__ void __launch_bounds__(kBlock)
+    k_nlist(int n, const float4 *__restrict__ x, const float4 *__restrict__ off,
+            const float4 *__restrict__ bnd4, const int *__restrict__ cs,
+            const int *__restrict__ bcs, DevGrid g, SweepConst sc, float skin,
+            int4 *__restrict__ meta, int *__restrict__ glist, int *__restrict__ blist,
+            int remap) {
+  const int i = xcd_block(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 bi = x[i];
+  const float4 oi = off[i];
+  int bx = 0, by = 0, bz = 0;
+  const bool ok = trunc_cell(bi.x + oi.x, bi.y + oi.y, bi.z + oi.z, g, bx, by, bz);
+  const int ext = ext_base_id(ok, bx, by, bz, g);
+  const float rbi = fmaxf(bi.w, sc.rmin);
+  const float blim = (sc.rb + rbi) + skin;
+  int gc = 0, bc = 0;
+  if (ext >= 0) {
+    stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
+      for (int k = k0; k < k1; ++k) {
+        const float4 q = bnd4[k];
+        const float px = (bi.x - q.x) + oi.x;
+        const float py = (bi.y - q.y) + oi.y;
+        const float pz = (bi.z - q.z) + oi.z;
+        const float d2 = px * px + py * py + pz * pz;
+        if (d2 < blim * blim) {
+          if (bc < sc.bmax) blist[(size_t)bc * sc.cap + i] = k;
+          ++bc;
+        }
+      }
+    });
+    auto test = [&](int j, const float4 &bj, const float4 &oj) {
+      const float px = (bi.x - bj.x) + (oi.x - oj.x);
+      const float py = (bi.y - bj.y) + (oi.y - oj.y);
+      const float pz = (bi.z - bj.z) + (oi.z - oj.z);
+      const float d2 = px * px + py * py + pz * pz;
+      const float lim = (bi.w + bj.w) + skin;
+      if (d2 < lim * lim && j != i) {
+        if (gc < sc.gmax) glist[(size_t)gc * sc.cap + i] = j;
+        ++gc;
+      }
+    };
+    stencil_rows(bx, by, bz, g, cs, [&](int j0, int j1) {
+      int j = j0;
+      for (; j + 4 <= j1; j += 4) {
+        const float4 b0 = x[j], b1 = x[j + 1], b2 = x[j + 2], b3 = x[j + 3];
+        const float4 o0 = off[j], o1 = off[j + 1], o2 = off[j + 2], o3 = off[j + 3];
+        test(j, b0, o0);
+        test(j + 1, b1, o1);
+        test(j + 2, b2, o2);
+        test(j + 3, b3, o3);
+      }
+      for (; j < j1; ++j) test(j, x[j], off[j]);
+    });
+  }
+  meta[i] = make_int4(gc, bc, ext, hash_cell(bi.x, bi.y, bi.z, g));
+}
+
+struct SweepArgs {
+  int n;
+  const float4 *__restrict__ base;
+  const float4 *__restrict__ off_in;
+  float4 *__restrict__ off_out;
+  const int4 *__restrict__ meta;
+  const int *__restrict__ glist;
+  const int *__restrict__ blist;
+  const float4 *__restrict__ bnd4;
+  const int *__restrict__ cs;
+  const int *__restrict__ bcs;
+  int *__restrict__ dirty;
+  const float4 *__restrict__ d0;
+  float4 *__restrict__ xb_out;
+  int *__restrict__ stats;    // full-stencil particle count of this sweep
+  DevGrid g;
+  SweepConst sc;
+  int use_lists;
+  int remap;
+};
+
+// Shared epilogue: new offset (and for the last stabilisation sweep the
+// position base), dirty flags when the particle first passes half the
+// valid skin.
+template <bool LAST_STAB>
+__device__ __forceinline__ void sweep_store(const SweepArgs &a, int i, const float4 &bi,
+                                            const float4 &oi, float3 ab, float3 ag, int cnt,
+                                            int cell) {
+  const float nf = (float)(cnt > 1 ? cnt : 1);
+  const float ix = (ab.x + ag.x) / nf;
+  const float iy = (ab.y + ag.y) / nf;
+  const float iz = (ab.z + ag.z) / nf;
+  const float ox = oi.x + ix, oy = oi.y + iy, oz = oi.z + iz;
+  const float cum_new = oi.w + sqrtf(ix * ix + iy * iy + iz * iz);
+  if (a.use_lists && oi.w < a.sc.half_valid && !(cum_new < a.sc.half_valid)) {
+    // first sweep past half the valid skin: flag the stencil around the
+    // particle's cell so that neighbours stop trusting their lists.
+    const DevGrid &g = a.g;
+    if (cell >= 0 && cell < g.cx * g.cy * g.cz) {
+      const int cz0 = cell % g.cz, cy0 = (cell / g.cz) % g.cy, cx0 = cell / (g.cz * g.cy);
+      for (int m = 0; m < 27; ++m) {
+        const int cc = stencil_cell(cx0, cy0, cz0, m, g);
+        if (cc >= 0) a.dirty[cc] = 1;
+      }
+    }
+  }
+  if (LAST_STAB) {
+    const float4 d = a.d0[i];
+    a.xb_out[i] = make_float4(bi.x + (ox - d.x), bi.y + (oy - d.y), bi.z + (oz - d.z), bi.w);
+    a.off_out[i] = make_float4(d.x, d.y, d.z, cum_new);
+  } else {
+    a.off_out[i] = make_float4(ox, oy, oz, cum_new);
+  }
+}
+
+// Full 27-cell stencil evaluation for particle i (row order = scan order).
+template <bool FRIC>
+__device__ __forceinline__ void full_stencil(const SweepArgs &a, int i, const float4 &bi,
+                                             const float4 &oi, int bx, int by, int bz, float rcb,
+                                             float preb, float3 &ab, float3 &ag, int &nb,
+                                             int &ng) {
+  const SweepConst &sc = a.sc;
+  stencil_rows(bx, by, bz, a.g, a.bcs, [&](int k0, int k1) {
+    for (int k = k0; k < k1; ++k) boundary_contact<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, ab, nb);
+  });
+  stencil_rows(bx, by, bz, a.g, a.cs, [&](int j0, int j1) {
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
+      const float4 b0 = a.base[j], b1 = a.base[j + 1], b2 = a.base[j + 2], b3 = a.base[j + 3];
+      const float4 o0 = a.off_in[j], o1 = a.off_in[j + 1], o2 = a.off_in[j + 2],
+                   o3 = a.off_in[j + 3];
+      if (j != i) granular_contact<FRIC>(bi, oi, b0, o0, sc, ag, ng);
+      if (j + 1 != i) granular_contact<FRIC>(bi, oi, b1, o1, sc, ag, ng);
+      if (j + 2 != i) granular_contact<FRIC>(bi, oi, b2, o2, sc, ag, ng);
+      if (j + 3 != i) granular_contact<FRIC>(bi, oi, b3, o3, sc, ag, ng);
+    }
+    for (; j < j1; ++j)
+      if (j != i) granular_contact<FRIC>(bi, oi, a.base[j], a.off_in[j], sc, ag, ng);
+  });
+}
+
+// One sweep.  Particles whose candidate list is provably exact for this
+// sweep use it; the others scan the full stencil.  Both visit contacts in
+// the same order, so the result does not depend on which path ran.
+template <bool FRIC, bool LAST_STAB>
+__global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
+  const int i = xcd_block(blockIdx.x, gridDim.x, a.remap) * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const DevGrid &g = a.g;
+  const SweepConst &sc = a.sc;
+  const float4 bi = a.base[i];
+  const float4 oi = a.off_in[i];
+  int bx = 0, by = 0, bz = 0;
+  const bool ok = trunc_cell(bi.x + oi.x, bi.y + oi.y, bi.z + oi.z, g, bx, by, bz);
+  const int ext = ext_base_id(ok, bx, by, bz, g);
+  bool list_ok = false;
+  int4 mt = make_int4(0, 0, -1, -1);
+  if (a.use_lists) {
+    mt = a.meta[i];
+    const int bcell = base_cell_id(ok, bx, by, bz, g);
+    list_ok = ext == mt.z && mt.x <= sc.gmax && mt.y <= sc.bmax && oi.w < sc.half_valid &&
+              bcell >= 0 && a.dirty[bcell] == 0;
+  }
+  const float rcb = sc.rb + fmaxf(bi.w, sc.rmin);
+  const float preb = contact_prefilter(rcb);
+  float3 ab = make_float3(0.0f, 0.0f, 0.0f), ag = make_float3(0.0f, 0.0f, 0.0f);
+  int nb = 0, ng = 0;
+  if (list_ok) {
+    for (int t = 0; t < mt.y; ++t) {
+      const int k = a.blist[(size_t)t * sc.cap + i];
+      boundary_contact<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, ab, nb);
+    }
+    // candidates in list order, gathered four at a time
+    const int *gl = a.glist + i;
+    const size_t stride = (size_t)sc.cap;
+    int t = 0;
+    for (; t + 4 <= mt.x; t += 4) {
+      const int j0 = gl[(t + 0) * stride], j1 = gl[(t + 1) * stride];
+      const int j2 = gl[(t + 2) * stride], j3 = gl[(t + 3) * stride];
+      const float4 b0 = a.base[j0], b1 = a.base[j1], b2 = a.base[j2], b3 = a.base[j3];
+      const float4 o0 = a.off_in[j0], o1 = a.off_in[j1], o2 = a.off_in[j2], o3 = a.off_in[j3];
+      granular_contact<FRIC>(bi, oi, b0, o0, sc, ag, ng);
+      granular_contact<FRIC>(bi, oi, b1, o1, sc, ag, ng);
+      granular_contact<FRIC>(bi, oi, b2, o2, sc, ag, ng);
+      granular_contact<FRIC>(bi, oi, b3, o3, sc, ag, ng);
+    }
+    for (; t < mt.x; ++t) {
+      const int j = gl[t * stride];
+      granular_contact<FRIC>(bi, oi, a.base[j], a.off_in[j], sc, ag, ng);
+    }
+  } else {
+    if (a.stats) {
+      // count particles that scanned the full stencil (one atomic per wave)
+      const unsigned long long fb = __ballot(1);
+      if ((threadIdx.x & 63) == __ffsll((long long)fb) - 1) atomicAdd(a.stats, (int)__popcll(fb));
+    }
+    if (ext >= 0) full_stencil<FRIC>(a, i, bi, oi, bx, by, bz, rcb, preb, ab, ag, nb, ng);
+  }
+  sweep_store<LAST_STAB>(a, i, bi, oi, ab, ag, nb + ng, mt.w);
+}
+
+struct FinalConst {
+  float inv_dt, sleep, damp_speed, damping, adaptive_damping;
+};
+
+__global__ void __launch_bounds__(kBlock)
+    k_finalize(int n, const float4 *__restrict__ base, const float4 *__restrict__ off,
+               float *__restrict__ pos, float *__restrict__ vel, int *__restrict__ adl,
+               const float *__restrict__ mass_tmp, float *__restrict__ mass,
+               const int *__restrict__ rem_tmp, int *__restrict__ rem,
+               const float *__restrict__ mer_tmp, float *__restrict__ mer,
+               const int *__restrict__ mc_tmp, int *__restrict__ mc, FinalConst fc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 b = base[i];
+  const float4 o = off[i];
+  float vx = fc.inv_dt * o.x, vy = fc.inv_dt * o.y, vz = fc.inv_dt * o.z;
+  const float speed = sqrtf(vx * vx + vy * vy + vz * vz);
+  if (speed < fc.sleep) {
+    vx = 0.0f;
+    vy = 0.0f;
+    vz = 0.0f;
+  } else if (speed > fc.damp_speed) {
+    const float e = exp_f(-speed * 0.1f);
+    const float f = fc.damping + (1.0f - fc.damping) * e;
+    vx = vx * f;
+    vy = vy * f;
+    vz = vz * f;
+    if (adl && adl[i] == 1) {
+      vx = vx * fc.adaptive_damping;
+      vy = vy * fc.adaptive_damping;
+      vz = vz * fc.adaptive_damping;
+    }
+  }
+  vel[3 * i] = vx;
+  vel[3 * i + 1] = vy;
+  vel[3 * i + 2] = vz;
+  pos[3 * i] = b.x + o.x;
+  pos[3 * i + 1] = b.y + o.y;
+  pos[3 * i + 2] = b.z + o.z;
+  if (adl) adl[i] = 0;
+  if (mass) mass[i] = mass_tmp[i];
+  if (rem) rem[i] = rem_tmp[i];
+  if (mer) mer[i] = mer_tmp[i];
+  if (mc) mc[i] = mc_tmp[i];
+}
+
+static inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+static int check_params(const gg_params *p, const gg_workspace *ws) {
+  if (!p) return GG_ERR_INVALID;
+  if (p->stab_iters < 0 || p->solver_iters < 0) return GG_ERR_INVALID;
+  if (p->stab_iters + p->solver_iters > ws->max_sweeps) return GG_ERR_CAPACITY;
+  if (!(p->dt > 0.0f)) return GG_ERR_INVALID;
+  return GG_OK;
+}
+
+static bool lists_on(const gg_params *p, const gg_workspace *ws) {
+  return p->skin > 0.0f && !(p->flags & GG_FLAG_NO_NLIST) && ws->gmax > 0;
+}
+
+static SweepConst sweep_const(const gg_params *p, const gg_workspace *ws) {
+  SweepConst sc;
+  sc.rb = p->boundary_radius;
+  sc.rmin = p->contact_radius_min;
+  sc.mu_s = p->mu_s;
+  sc.mu_k = p->mu_k;
+  sc.half_valid = 0.5f * kListValidFrac * p->skin;
+  sc.gmax = ws->gmax;
+  sc.bmax = ws->bmax;
+  sc.cap = ws->capacity + 1;
+  return sc;
+}
+
+// permute (perm may be null = identity) + predict + lists.
+static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, const int *key,
+                       const int *cs, const int *bcs, int nbnd, const gg_params *p,
+                       const gg_workspace *ws, hipStream_t st) {
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  const int n = set->n;
+  const int ndirty = (int)(grid_cells(ws->grid) + 1);
+  const float dvx = p->dt * p->gx, dvy = p->dt * p->gy, dvz = p->dt * p->gz;
+  const double denom = 4.0 * kPiD * (double)(int)p->density;
+  const int nblk = n > 0 ? blocks_for(n) : 1;
+  prof_begin(GG_STAGE_PREDICT, st);
+  k_permute_predict<<<nblk, kBlock, 0, st>>>(
+      n, perm, key, reinterpret_cast<const float *>(set->pos),
+      reinterpret_cast<const float *>(set->vel), set->mass, at<float4>(ws, L.x0),
+      at<float4>(ws, L.off0), at<float4>(ws, L.d0), at<float>(ws, L.ftmp0), set->scaled_mass,
+      set->particle_2_cell, ad ? ad->remove : nullptr, ad ? ad->merge : nullptr,
+      ad ? ad->merge_count : nullptr, at<int>(ws, L.itmp0), at<float>(ws, L.ftmp1),
+      at<int>(ws, L.itmp1), p->dt, dvx, dvy, dvz, denom, at<int>(ws, L.dirty), ndirty,
+      at<int>(ws, L.ctr) + 1);
+  GG_LAUNCH_CHECK();
+  prof_end(GG_STAGE_PREDICT, st);
+  if (lists_on(p, ws) && n > 0) {
+    prof_begin(GG_STAGE_NLIST, st);
+    k_nlist<<<blocks_for(n), kBlock, 0, st>>>(
+        n, at<float4>(ws, L.x0), at<float4>(ws, L.off0), at<float4>(ws, L.bnd4), cs, bcs,
+        dev_grid(ws->grid), sweep_const(p, ws), p->skin, at<int4>(ws, L.meta),
+        at<int>(ws, L.glist), at<int>(ws, L.blist), (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1);
+    GG_LAUNCH_CHECK();
+    prof_end(GG_STAGE_NLIST, st);
+  }
+  (void)nbnd;
+  return GG_OK;
+}
+
+template <bool FRIC, bool LAST>
+static int launch_sweep(const SweepArgs &a, int nblocks, hipStream_t st) {
+  k_sweep<FRIC, LAST><<<nblocks, kBlock, 0, st>>>(a);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+
+static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, const gg_params *p,
+                     int k, const gg_workspace *ws, hipStream_t st) {
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  const int n = set->n;
+  if (n <= 0) return GG_OK;
+  const int S = p->stab_iters;
+  const bool last_stab = (k == S - 1);
+  const bool fric = (k >= S) && (p->mu_s > 0.0f || p->mu_k > 0.0f);
+  SweepArgs a;
+  a.n = n;
+  a.base = (k < S || S == 0) ? at<float4>(ws, L.x0) : at<float4>(ws, L.xb);
+  a.off_in = (k % 2 == 0) ? at<float4>(ws, L.off0) : at<float4>(ws, L.off1);
+  a.off_out = (k % 2 == 0) ? at<float4>(ws, L.off1) : at<float4>(ws, L.off0);
+  a.meta = at<int4>(ws, L.meta);
+  a.glist = at<int>(ws, L.glist);
+  a.blist = at<int>(ws, L.blist);
+  a.bnd4 = at<float4>(ws, L.bnd4);
+  a.cs = cs;
+  a.bcs = bcs;
+  a.dirty = at<int>(ws, L.dirty);
+  a.d0 = at<float4>(ws, L.d0);
+  a.xb_out = at<float4>(ws, L.xb);
+  a.stats = at<int>(ws, L.ctr) + 1 + (k % kFallbackSlots);
+  a.g = dev_grid(ws->grid);
+  a.sc = sweep_const(p, ws);
+  a.use_lists = lists_on(p, ws) ? 1 : 0;
+  a.remap = (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1;
+  const int nb = blocks_for(n);
+  prof_begin(GG_STAGE_SWEEP, st);
+  int rc;
+  if (last_stab)
+    rc = launch_sweep<false, true>(a, nb, st);
+  else if (fric)
+    rc = launch_sweep<true, false>(a, nb, st);
+  else
+    rc = launch_sweep<false, false>(a, nb, st);
+  if (rc != GG_OK) return rc;
+  prof_end(GG_STAGE_SWEEP, st);
+  return GG_OK;
+}
+
+static int run_finalize(gg_particles *set, gg_adaptive *ad, const gg_params *p,
+                        const gg_workspace *ws, hipStream_t st) {
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  const int n = set->n;
+  if (n <= 0) return GG_OK;
+  const int S = p->stab_iters, I = p->solver_iters;
+  const float4 *base = S > 0 ? at<float4>(ws, L.xb) : at<float4>(ws, L.x0);
+  const float4 *off = ((S + I) % 2 == 0) ? at<float4>(ws, L.off0) : at<float4>(ws, L.off1);
+  FinalConst fc;
+  fc.inv_dt = 1.0f / p->dt;
+  fc.sleep = p->sleep_speed;
+  fc.damp_speed = p->damp_speed;
+  fc.damping = p->damping;
+  fc.adaptive_damping = p->adaptive_damping;
+  prof_begin(GG_STAGE_FINALIZE, st);
+  k_finalize<<<blocks_for(n), kBlock, 0, st>>>(
+      n, base, off, reinterpret_cast<float *>(set->pos), reinterpret_cast<float *>(set->vel),
+      set->adaptive_last_step, at<float>(ws, L.ftmp0), set->mass, at<int>(ws, L.itmp0),
+      ad ? ad->remove : nullptr, at<float>(ws, L.ftmp1), ad ? ad->merge : nullptr,
+      at<int>(ws, L.itmp1), ad ? ad->merge_count : nullptr, fc);
+  GG_LAUNCH_CHECK();
+  prof_end(GG_STAGE_FINALIZE, st);
+  return GG_OK;
+}
+
+static int check_set(const gg_particles *set, const gg_workspace *ws) {
+  if (!set) return GG_ERR_INVALID;
+  if (set->n < 0) return GG_ERR_INVALID;
+  if (set->n > ws->capacity) return GG_ERR_CAPACITY;
+  if (set->n > 0 && (!set->pos || !set->vel || !set->mass)) return GG_ERR_INVALID;
+  return GG_OK;
+}
+
+static int check_boundary(const gg_particles *b, const int *bcs, const gg_workspace *ws) {
+  if (!b || !bcs) return GG_ERR_INVALID;
+  if (b->n < 0) return GG_ERR_INVALID;
+  if (b->n > ws->boundary_capacity) return GG_ERR_CAPACITY;
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" int gg_step(gg_particles *set, gg_adaptive *ad, int *cell_start,
+                       const gg_particles *boundary, const int *bcell_start,
+                       const gg_params *params, const gg_workspace *ws, hipStream_t stream) {
+  GG_TRY(check_ws(ws));
+  GG_TRY(check_params(params, ws));
+  GG_TRY(check_set(set, ws));
+  GG_TRY(check_boundary(boundary, bcell_start, ws));
+  if (!cell_start) return GG_ERR_INVALID;
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  prof_begin(GG_STAGE_SORT, stream);
+  GG_TRY(sort_by_cell(reinterpret_cast<const float *>(set->pos), set->n, cell_start, ws, stream));
+  prof_end(GG_STAGE_SORT, stream);
+  GG_TRY(run_predict(set, ad, at<int>(ws, L.perm), at<int>(ws, L.key), cell_start, bcell_start,
+                     boundary->n, params, ws, stream));
+  const int K = params->stab_iters + params->solver_iters;
+  for (int k = 0; k < K; ++k)
+    GG_TRY(run_sweep(set, cell_start, bcell_start, params, k, ws, stream));
+  return run_finalize(set, ad, params, ws, stream);
+}
+
+extern "C" int gg_predict(gg_particles *set, const int *cell_start, const gg_particles *boundary,
+                          const int *bcell_start, const gg_params *params,
+                          const gg_workspace *ws, hipStream_t stream) {
+  GG_TRY(check_ws(ws));
+  GG_TRY(check_params(params, ws));
+  GG_TRY(check_set(set, ws));
+  GG_TRY(check_boundary(boundary, bcell_start, ws));
+  if (!cell_start) return GG_ERR_INVALID;
+  return run_predict(set, nullptr, nullptr, nullptr, cell_start, bcell_start, boundary->n, params,
+                     ws, stream);
+}
+
+extern "C" int gg_contact_sweep(gg_particles *set, const int *cell_start,
+                                const gg_particles *boundary, const int *bcell_start,
+                                const gg_params *params, int sweep, const gg_workspace *ws,
+                                hipStream_t stream) {
+  GG_TRY(check_ws(ws));
+  GG_TRY(check_params(params, ws));
+  GG_TRY(check_set(set, ws));
+  GG_TRY(check_boundary(boundary, bcell_start, ws));
+  if (!cell_start) return GG_ERR_INVALID;
+  if (sweep < 0 || sweep >= params->stab_iters + params->solver_iters) return GG_ERR_INVALID;
+  return run_sweep(set, cell_start, bcell_start, params, sweep, ws, stream);
+}
+
+extern "C" int gg_finalize(gg_particles *set, const gg_params *params, const gg_workspace *ws,
+                           hipStream_t stream) {
+  GG_TRY(check_ws(ws));
+  GG_TRY(check_params(params, ws));
+  GG_TRY(check_set(set, ws));
+  return run_finalize(set, nullptr, params, ws, stream);
+}
+
+extern "C" const void *gg_workspace_view(const gg_workspace *ws, int which) {
+  if (check_ws(ws) != GG_OK) return nullptr;
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  switch (which) {
+    case 0: return at<void>(ws, L.x0);
+    case 1: return at<void>(ws, L.off0);
+    case 2: return at<void>(ws, L.meta);
+    case 3: return at<void>(ws, L.glist);
+    case 4: return at<void>(ws, L.blist);
+    case 5: return at<void>(ws, L.off1);
+    case 6: return at<void>(ws, L.xb);
+    case 7: return at<void>(ws, L.d0);
+    case 8: return at<void>(ws, L.ctr);
+    default: return nullptr;
+  }
+}

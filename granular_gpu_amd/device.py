@@ -1,0 +1,142 @@
+"""Device-side driver over the C-ABI: owns the device arrays of one scene
+(torch tensors used purely as device allocations) and calls the library's
+stage and step entry points on torch's current HIP stream.
+
+Array roles mirror the reference's GranularSystem (GranularSystem.hpp:60-90):
+a granular set, a static boundary set, their cell tables and a workspace.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, load
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class PlainSim:
+    """Plain (non-adaptive) scene on one GPU."""
+
+    def __init__(self, scene, stab=3, solver=5, mu_s=0.0, mu_k=0.0, skin=0.005, gmax=48,
+                 bmax=16, flags=0, capacity=None, device="cuda", max_sweeps=None):
+        lib = load()
+        self.lib = lib
+        self.scene = scene
+        dev = torch.device(device)
+        self.dev = dev
+        n = scene.n
+        cap = capacity or n
+        nb = scene.boundary.shape[0]
+        self.n, self.cap, self.nb = n, cap, nb
+        f32, i32 = torch.float32, torch.int32
+        self.pos = torch.zeros((cap, 3), dtype=f32, device=dev)
+        self.vel = torch.zeros((cap, 3), dtype=f32, device=dev)
+        self.mass = torch.zeros(cap, dtype=f32, device=dev)
+        self.scaled_mass = torch.zeros(cap, dtype=f32, device=dev)
+        self.p2c = torch.zeros(cap, dtype=i32, device=dev)
+        self.adl = torch.zeros(cap, dtype=i32, device=dev)
+        self.is_animated = torch.zeros(cap, dtype=i32, device=dev)
+        self.pos[:n] = torch.from_numpy(scene.pos).to(dev)
+        self.vel[:n] = torch.from_numpy(scene.vel).to(dev)
+        self.mass[:n] = torch.from_numpy(scene.mass).to(dev)
+        self.bpos = torch.from_numpy(np.ascontiguousarray(scene.boundary)).to(dev).contiguous()
+        if nb == 0:
+            self.bpos = torch.zeros((1, 3), dtype=f32, device=dev)
+        self.bvel = torch.zeros_like(self.bpos)
+        self.bmass = torch.ones(self.bpos.shape[0], dtype=f32, device=dev)
+        self.bp2c = torch.zeros(self.bpos.shape[0], dtype=i32, device=dev)
+        self.banim = torch.zeros(self.bpos.shape[0], dtype=i32, device=dev)
+        cx, cy, cz = scene.cells
+        self.C = cx * cy * cz
+        self.cs = torch.zeros(self.C + 2, dtype=i32, device=dev)
+        self.bcs = torch.zeros(self.C + 2, dtype=i32, device=dev)
+        self.grid = _lib.gg_grid(float(scene.cell_length), cx, cy, cz)
+        p = _lib.default_params()
+        p.dt = float(scene.dt)
+        p.gx, p.gy, p.gz = (float(v) for v in scene.gravity)
+        p.density = float(scene.density)
+        p.stab_iters, p.solver_iters = stab, solver
+        p.mu_s, p.mu_k = mu_s, mu_k
+        p.skin = skin
+        p.flags = flags
+        self.params = p
+        ms = max_sweeps or (stab + solver)
+        nbytes = lib.gg_workspace_bytes(cap, max(nb, 1), self.grid, ms, gmax, bmax)
+        if nbytes == 0:
+            raise ValueError("invalid workspace request")
+        self.ws_buf = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        self.ws = _lib.gg_workspace(self.ws_buf.data_ptr(), int(nbytes), cap, max(nb, 1),
+                                    self.grid, ms, gmax, bmax)
+        check("gg_workspace_init", lib.gg_workspace_init(C.byref(self.ws), _stream()))
+        self.set = _lib.gg_particles()
+        self._refresh_set()
+        self.bset = _lib.gg_particles(nb, self.bpos.shape[0], _ptr(self.bpos), _ptr(self.bvel),
+                                      _ptr(self.bmass), None, None, None, None, _ptr(self.bp2c),
+                                      _ptr(self.banim), None)
+        check("gg_boundary_build",
+              lib.gg_boundary_build(C.byref(self.bset), _ptr(self.bcs), C.byref(self.ws),
+                                    _stream()))
+
+    def _refresh_set(self):
+        s = self.set
+        s.n, s.capacity = self.n, self.cap
+        s.pos, s.vel, s.mass = _ptr(self.pos), _ptr(self.vel), _ptr(self.mass)
+        s.scaled_mass = _ptr(self.scaled_mass)
+        s.particle_2_cell = _ptr(self.p2c)
+        s.is_animated = _ptr(self.is_animated)
+        s.adaptive_last_step = _ptr(self.adl)
+
+    def step(self, nsteps=1):
+        for _ in range(nsteps):
+            check("gg_step", self.lib.gg_step(C.byref(self.set), None, _ptr(self.cs),
+                                              C.byref(self.bset), _ptr(self.bcs),
+                                              C.byref(self.params), C.byref(self.ws), _stream()))
+
+    # stage-level entry points ------------------------------------------------
+    def grid_build(self, perm_out: torch.Tensor | None = None):
+        check("gg_grid_build", self.lib.gg_grid_build(C.byref(self.set), None, _ptr(self.cs),
+                                                      _ptr(perm_out), C.byref(self.ws),
+                                                      _stream()))
+
+    def predict(self):
+        check("gg_predict", self.lib.gg_predict(C.byref(self.set), _ptr(self.cs),
+                                                C.byref(self.bset), _ptr(self.bcs),
+                                                C.byref(self.params), C.byref(self.ws),
+                                                _stream()))
+
+    def sweep(self, k):
+        check("gg_contact_sweep",
+              self.lib.gg_contact_sweep(C.byref(self.set), _ptr(self.cs), C.byref(self.bset),
+                                        _ptr(self.bcs), C.byref(self.params), k,
+                                        C.byref(self.ws), _stream()))
+
+    def finalize(self):
+        check("gg_finalize", self.lib.gg_finalize(C.byref(self.set), C.byref(self.params),
+                                                  C.byref(self.ws), _stream()))
+
+    def view(self, which: int, count: int, dtype=torch.float32, width=4) -> torch.Tensor:
+        """Copy of a workspace buffer (see gg_workspace_view)."""
+        ptr = self.lib.gg_workspace_view(C.byref(self.ws), which)
+        if not ptr:
+            raise RuntimeError("invalid workspace view")
+        off = ptr - self.ws_buf.data_ptr()
+        esz = torch.empty((), dtype=dtype).element_size()
+        raw = self.ws_buf[off: off + count * width * esz]
+        return raw.view(dtype).reshape(count, width).clone()
+
+    # host copies -------------------------------------------------------------
+    def host(self):
+        torch.cuda.synchronize()
+        n = self.n
+        return (self.pos[:n].cpu().numpy().copy(), self.vel[:n].cpu().numpy().copy(),
+                self.mass[:n].cpu().numpy().copy())

@@ -1,0 +1,208 @@
+/*
+ * granular_gpu.h -- C-ABI of the MI355X-native granular PBD solver.
+ *
+ * Every entry point is extern "C", takes plain device pointers, sizes and
+ * small parameter structs, returns GG_OK (0) or a negative GG_ERR_* code, and
+ * never throws.  Nothing here allocates device memory: scratch space is a
+ * caller-owned workspace sized by gg_workspace_bytes().  All work is enqueued
+ * on the given HIP stream (NULL = the null stream) and is asynchronous unless
+ * stated otherwise.
+ *
+ * The reference (stingyemperor/granular_gpu) has no C-ABI of its own: its
+ * boundary is the C++ class API (GranularSystem / Solver / GranularParticles /
+ * DArray) whose methods launch raw-pointer CUDA kernels.  Each entry point
+ * below names the reference functions and kernels it stands in for.
+ *
+ * Array conventions (same as the reference):
+ *   - positions / velocities are packed float3 (12 bytes, AoS);
+ *   - a cell table has C+2 ints for a grid of C = cx*cy*cz cells:
+ *     cell_start[c] for c in [0, C] is the reference's cellStart array
+ *     (entry C is the sentinel cell of particles outside the grid) and
+ *     cell_start[C+1] = n;
+ *   - cell ids are x-major: ((ix*cy)+iy)*cz+iz with ix = floor(x/L).
+ */
+#ifndef GRANULAR_GPU_H
+#define GRANULAR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same type as hip_runtime_api.h's hipStream_t. */
+typedef struct ihipStream_t *hipStream_t;
+
+enum {
+  GG_OK = 0,
+  GG_ERR_INVALID = -1,   /* bad argument (NULL pointer, negative size ...) */
+  GG_ERR_CAPACITY = -2,  /* n exceeds the capacity the workspace was sized for */
+  GG_ERR_HIP = -3,       /* a HIP runtime call failed */
+  GG_ERR_WORKSPACE = -4, /* workspace too small or not initialised */
+  GG_ERR_NO_DEVICE = -5  /* no usable gfx950 device */
+};
+
+typedef struct gg_float3 {
+  float x, y, z;
+} gg_float3;
+
+/* Uniform grid: cell length L and cell counts.  Reference: _cell_length,
+ * _cell_size (GranularSystem.hpp:247-253), cell_size = ceil(space/L)
+ * (main.cu:158-160). */
+typedef struct gg_grid {
+  float cell_length;
+  int cx, cy, cz;
+} gg_grid;
+
+/* Step parameters.  gg_params_default() fills the reference's constants. */
+typedef struct gg_params {
+  float dt;                 /* time step (scene JSON "dt")                 */
+  float gx, gy, gz;         /* gravity (scene JSON "gravity")              */
+  float density;            /* scene JSON "density"                         */
+  int stab_iters;           /* stabilisation sweeps, reference 3            */
+  int solver_iters;         /* solver sweeps, reference 5                   */
+  float mu_s, mu_k;         /* PBD friction; 0,0 reproduces the reference   */
+  float boundary_radius;    /* 0.01  (Solver.cu:283)                        */
+  float contact_radius_min; /* 0.01  (Solver.cu:281)                        */
+  float sleep_speed;        /* 0.01  (Solver.cu:223)                        */
+  float damp_speed;         /* 1.0   (Solver.cu:202)                        */
+  float damping;            /* 0.99  (Solver.cu:204-205)                    */
+  float adaptive_damping;   /* 0.98  (Solver.cu:242)                        */
+  float skin;               /* neighbour-list skin; <= 0 disables lists     */
+  int flags;                /* GG_FLAG_*                                    */
+} gg_params;
+
+#define GG_FLAG_NO_NLIST 1     /* always scan the 27-cell stencil */
+#define GG_FLAG_NO_XCD_REMAP 2 /* keep hardware block order (no XCD grouping) */
+
+/* Device arrays of one particle set.  Mirrors GranularParticles
+ * (GranularParticles.hpp:5-110) + Particles (Particles.hpp:7-62).
+ * Arrays that a stage does not use may be NULL. */
+typedef struct gg_particles {
+  int n;
+  int capacity;
+  gg_float3 *pos;
+  gg_float3 *vel;
+  float *mass;
+  float *scaled_mass;
+  int *surface;
+  float *surface_distance;
+  int *num_surface;
+  int *particle_2_cell;
+  int *is_animated;
+  int *adaptive_last_step;
+} gg_particles;
+
+/* Merge/split bookkeeping.  Mirrors Solver's _buffer_remove, _buffer_merge,
+ * _buffer_merge_velocity, _buffer_merge_count (Solver.hpp:108-112). */
+typedef struct gg_adaptive {
+  int *remove;
+  float *merge;
+  gg_float3 *merge_velocity;
+  int *merge_count;
+} gg_adaptive;
+
+/* Caller-owned scratch.  Size with gg_workspace_bytes(), allocate device
+ * memory of that size, fill the struct, call gg_workspace_init() once. */
+typedef struct gg_workspace {
+  void *base;            /* device pointer, 256-byte aligned */
+  size_t bytes;
+  int capacity;          /* max particles of the granular set */
+  int boundary_capacity; /* max particles of the boundary set */
+  gg_grid grid;
+  int max_sweeps;        /* >= stab_iters + solver_iters */
+  int gmax, bmax;        /* neighbour-list widths */
+} gg_workspace;
+
+/* ---- library ------------------------------------------------------------ */
+const char *gg_version(void);
+const char *gg_status_string(int status);
+/* Current device is a gfx950 (MI355X).  Returns GG_OK or GG_ERR_NO_DEVICE. */
+int gg_device_check(void);
+void gg_params_default(gg_params *p);
+
+/* ---- workspace ---------------------------------------------------------- */
+size_t gg_workspace_bytes(int capacity, int boundary_capacity, gg_grid grid,
+                          int max_sweeps, int gmax, int bmax);
+/* Clears the counters the kernels rely on.  Stream-ordered. */
+int gg_workspace_init(const gg_workspace *ws, hipStream_t stream);
+
+/* ---- neighbour search --------------------------------------------------- */
+/* Hash + stable sort by cell id + cell table + in-place permutation of the
+ * set's pos, vel, mass, is_animated and (if ad != NULL) the merge/remove
+ * buffers.  Sorted keys go to particle_2_cell.  perm_out (optional, n ints)
+ * receives the old index of each sorted slot.
+ * Replaces: neighbor_search_granular / neighbor_search_boundary
+ * (GranularSystem.cu:65-280), mapParticles2Cells_CUDA + countingInCell_CUDA
+ * (CUDAFunctions.cuh:54-107), thrust sort_by_key / gather / exclusive_scan. */
+int gg_grid_build(gg_particles *set, gg_adaptive *ad, int *cell_start,
+                  int *perm_out, const gg_workspace *ws, hipStream_t stream);
+
+/* Boundary preparation: gg_grid_build on the boundary set, plus the packed
+ * copy the contact kernels read.  Call once per boundary change.
+ * Replaces: the GranularSystem ctor's neighbor_search_boundary
+ * (GranularSystem.cu:55) and the moving-boundary re-search (:402-404). */
+int gg_boundary_build(gg_particles *boundary, int *bcell_start,
+                      const gg_workspace *ws, hipStream_t stream);
+
+/* ---- plain PBD step ----------------------------------------------------- */
+/* Grid build + predict + stab/solver contact sweeps + finalize for the
+ * granular set, against a boundary prepared by gg_boundary_build.
+ * Replaces: GranularSystem::step's neighbor_search_granular and
+ * Solver::step (Solver.cu:82-109: add_external_force,
+ * update_particle_positions, apply_mass_scaling, project, final_update,
+ * adaptive_last_step reset). */
+int gg_step(gg_particles *set, gg_adaptive *ad, int *cell_start,
+            const gg_particles *boundary, const int *bcell_start,
+            const gg_params *params, const gg_workspace *ws,
+            hipStream_t stream);
+
+/* Stage entry points of gg_step, in order (for stage-level checking).
+ * gg_predict: T7-T9 (Solver.cu:125-191) on the sorted set; builds the
+ *   neighbour lists unless GG_FLAG_NO_NLIST.
+ * gg_contact_sweep: one compute_delta_pos + change_position pass
+ *   (Solver.cu:361-513); sweep in [0, stab_iters+solver_iters).
+ * gg_finalize: final_update + adaptive_last_step reset (Solver.cu:193-270,
+ *   103-108). */
+int gg_predict(gg_particles *set, const int *cell_start,
+               const gg_particles *boundary, const int *bcell_start,
+               const gg_params *params, const gg_workspace *ws,
+               hipStream_t stream);
+int gg_contact_sweep(gg_particles *set, const int *cell_start,
+                     const gg_particles *boundary, const int *bcell_start,
+                     const gg_params *params, int sweep,
+                     const gg_workspace *ws, hipStream_t stream);
+int gg_finalize(gg_particles *set, const gg_params *params,
+                const gg_workspace *ws, hipStream_t stream);
+
+/* ---- stage profiler ------------------------------------------------------ */
+/* When enabled, every stage launched by gg_step and the other entry points
+ * is bracketed by hipEvents on its stream (the reference times only the whole
+ * step, GranularSystem.cu:398-449).  gg_profiler_collect synchronises the events,
+ * adds each stage's elapsed milliseconds to total_ms[stage] and its launch
+ * count to count[stage] for stage < nstages, and clears the record list. */
+enum {
+  GG_STAGE_SORT = 0,     /* hash + counting sort + cell table */
+  GG_STAGE_PREDICT = 1,  /* permute + predict */
+  GG_STAGE_NLIST = 2,    /* neighbour candidate lists */
+  GG_STAGE_SWEEP = 3,    /* one contact sweep (stabilisation or solver) */
+  GG_STAGE_FINALIZE = 4, /* final update */
+  GG_STAGE_COUNT = 5
+};
+int gg_profiler_enable(int max_records); /* 0 disables */
+int gg_profiler_collect(double *total_ms, int *count, int nstages);
+
+/* Read-only views into the workspace for checking (device pointers):
+ * which = 0 base float4 (x,y,z,r) of the current phase, 1 offset float4
+ * (dx,dy,dz,cum) after the last sweep, 2 neighbour meta int4
+ * (gcount,bcount,base_cell,cell), 3 granular list [gmax][capacity],
+ * 4 boundary list [bmax][capacity], 5 offset pong, 6 base after
+ * stabilisation, 7 dt*v after gravity, 8 int counters: [1 + k % 62] =
+ * particles of sweep k that scanned the full stencil. */
+const void *gg_workspace_view(const gg_workspace *ws, int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRANULAR_GPU_H */

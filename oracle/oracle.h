@@ -1,0 +1,114 @@
+/*
+ * CPU oracle for the granular PBD step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This is a from-scratch C++/OpenMP restatement of the reference solver's
+ * algorithm (stingyemperor/granular_gpu, src/GranularSystem.cu, src/Solver.cu,
+ * src/CUDAFunctions.cuh), written to check the HIP implementation.  Only the
+ * test suite, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it.  The product library never links, loads or calls anything here.
+ *
+ * Two precisions:
+ *   *_f32 follows the exact FP32 operation order of the deterministic
+ *         semantics (no FMA contraction, IEEE division and square root), so a
+ *         conforming GPU implementation matches it bit for bit;
+ *   *_f64 runs the same algorithm in double precision and is the accuracy
+ *         reference for positions (1e-5*r after 1 step, 1e-3*r after 10).
+ *
+ * Array layout: float3 / double3 quantities are packed AoS (x,y,z per
+ * particle), exactly as the reference's float3 arrays (Particles.hpp:19-20).
+ * Cell tables have C+2 entries: cell_start[c] for c in [0, C] is the
+ * reference's cellStart (C = sentinel cell of out-of-grid particles) and
+ * cell_start[C+1] = n.
+ */
+#ifndef GRANULAR_ORACLE_H
+#define GRANULAR_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_params {
+  float dt;
+  float gx, gy, gz;
+  float cell_length;
+  int cx, cy, cz;
+  float density;            /* scene density; contacts use (int)density     */
+  int stab_iters;           /* reference: 3  (Solver.cu:428)                */
+  int solver_iters;         /* reference: 5  (Solver.cu:99)                 */
+  float mu_s, mu_k;         /* PBD friction; 0/0 = reference (inert clamp)  */
+  float boundary_radius;    /* 0.01 (Solver.cu:283)                          */
+  float contact_radius_min; /* 0.01 clamp of r_i vs boundary (Solver.cu:281) */
+  float sleep_speed;        /* 0.01 (Solver.cu:223)                          */
+  float damp_speed;         /* 1.0  (Solver.cu:202)                          */
+  float damping;            /* 0.99 (Solver.cu:204-205)                      */
+  float adaptive_damping;   /* 0.98 (Solver.cu:242)                          */
+  float skin;               /* neighbour-list skin (only for oracle_nlist)   */
+  int own_only;             /* 0: symmetric gather; 1: own term only        */
+} oracle_params;
+
+/* Grid build: hash, stable sort by cell id, cell_start (C+2 entries).
+ * perm[k] = old index of the particle placed at sorted slot k.            */
+int oracle_grid_build(const float *pos, int n, float cell_length, int cx,
+                      int cy, int cz, int *keys_sorted, int *perm,
+                      int *cell_start);
+int oracle_hash(const float *pos, int n, float cell_length, int cx, int cy,
+                int cz, int *keys);
+
+/* Full plain step(s) (grid build, predict, S+I contact sweeps, finalize) on
+ * host arrays, in place.  Boundary arrays must already be cell-sorted with
+ * their cell_start (C+2).  Optional outputs may be NULL.                   */
+int oracle_plain_steps_f32(int n, float *pos, float *vel, float *mass,
+                           int nb, const float *bpos, const int *bcell_start,
+                           const oracle_params *p, int nsteps,
+                           float *scaled_mass_out, int *p2c_out,
+                           int *adaptive_last_step);
+int oracle_plain_steps_f64(int n, double *pos, double *vel, double *mass,
+                           int nb, const double *bpos, const int *bcell_start,
+                           const oracle_params *p, int nsteps,
+                           int *adaptive_last_step, int *ids);
+
+/* Lockstep FP32 + FP64 run.  The FP64 state takes every discrete decision
+ * (sort order, stencil base, contact yes/no, friction branch, sleep and
+ * damping branch) from the FP32 state, so both follow the same discrete path
+ * and the difference is the rounding error of the FP32 arithmetic.  (An
+ * independent FP64 run can take different discrete decisions: pairs resolved
+ * to exactly d = r_i + r_j sit on the contact-count threshold.)            */
+int oracle_shadow_steps(int n, float *pos32, float *vel32, float *mass32,
+                        double *pos64, double *vel64, double *mass64, int nb,
+                        const float *bpos32, const double *bpos64,
+                        const int *bcell_start, const oracle_params *p,
+                        int nsteps, int *adaptive_last_step);
+
+/* One contact sweep in FP32 on explicit base/offset arrays (stage test).
+ * base4/off4 are float4 records (x,y,z,w): base w = radius.               */
+int oracle_sweep_f32(int n, const float *base4, const float *off4,
+                     float *off4_out, int nb, const float *bpos,
+                     const int *cell_start, const int *bcell_start,
+                     const oracle_params *p, int solver_phase);
+
+/* Neighbour candidate lists (granular + boundary) in scan order, as the GPU
+ * builds them: candidate iff in the stencil of trunc(xstar / L) and
+ * |p|^2 < lim^2 with lim = r_i + r_j + skin (granular) or
+ * R_b + max(r_i,rmin) + skin (boundary).  base_cell = padded stencil-base id.
+ * Lists are written row-major [i][k]; counts may exceed gmax/bmax (then
+ * only the first gmax/bmax entries are stored).                           */
+int oracle_nlist_f32(int n, const float *base4, const float *off4, int nb,
+                     const float *bpos, const int *cell_start,
+                     const int *bcell_start, const oracle_params *p, int gmax,
+                     int bmax, int *gcount, int *glist, int *bcount,
+                     int *blist, int *base_cell);
+
+/* Radius helpers (three reference variants), exposed for unit tests.     */
+float oracle_radius_contact(float m, float density);
+float oracle_radius_surface(float m, float density);
+float oracle_radius_split(float m, float density);
+float oracle_expf(float x);
+
+/* Number of OpenMP threads the oracle uses (0 = default).                */
+void oracle_set_threads(int t);
+int oracle_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

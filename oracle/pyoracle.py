@@ -1,0 +1,164 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "_build" / "liboracle.so"
+
+
+class oracle_params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float), ("gx", C.c_float), ("gy", C.c_float), ("gz", C.c_float),
+        ("cell_length", C.c_float), ("cx", C.c_int), ("cy", C.c_int), ("cz", C.c_int),
+        ("density", C.c_float), ("stab_iters", C.c_int), ("solver_iters", C.c_int),
+        ("mu_s", C.c_float), ("mu_k", C.c_float), ("boundary_radius", C.c_float),
+        ("contact_radius_min", C.c_float), ("sleep_speed", C.c_float),
+        ("damp_speed", C.c_float), ("damping", C.c_float), ("adaptive_damping", C.c_float),
+        ("skin", C.c_float), ("own_only", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", str(ORACLE_DIR)], check=True, capture_output=True)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    P = C.POINTER(oracle_params)
+    lib.oracle_hash.argtypes = [fp, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, ip]
+    lib.oracle_grid_build.argtypes = [fp, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, ip,
+                                      ip, ip]
+    lib.oracle_plain_steps_f32.argtypes = [C.c_int, fp, fp, fp, C.c_int, fp, ip, P, C.c_int,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.oracle_plain_steps_f64.argtypes = [C.c_int, dp, dp, dp, C.c_int, dp, ip, P, C.c_int,
+                                           C.c_void_p, C.c_void_p]
+    lib.oracle_shadow_steps.argtypes = [C.c_int, fp, fp, fp, dp, dp, dp, C.c_int, fp, dp, ip, P,
+                                        C.c_int, C.c_void_p]
+    lib.oracle_sweep_f32.argtypes = [C.c_int, fp, fp, fp, C.c_int, fp, ip, ip, P, C.c_int]
+    lib.oracle_nlist_f32.argtypes = [C.c_int, fp, fp, C.c_int, fp, ip, ip, P, C.c_int, C.c_int,
+                                     ip, ip, ip, ip, ip]
+    for nm in ("oracle_radius_contact", "oracle_radius_surface", "oracle_radius_split"):
+        getattr(lib, nm).argtypes = [C.c_float, C.c_float]
+        getattr(lib, nm).restype = C.c_float
+    lib.oracle_expf.argtypes = [C.c_float]
+    lib.oracle_expf.restype = C.c_float
+    lib.oracle_set_threads.argtypes = [C.c_int]
+    lib.oracle_get_threads.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def params_from(scene, stab=3, solver=5, mu_s=0.0, mu_k=0.0, skin=0.005,
+                own_only=0) -> oracle_params:
+    """Oracle parameters for a scenes.Scene (reference constants elsewhere)."""
+    p = oracle_params()
+    p.dt = float(scene.dt)
+    p.gx, p.gy, p.gz = (float(v) for v in scene.gravity)
+    p.cell_length = float(scene.cell_length)
+    p.cx, p.cy, p.cz = scene.cells
+    p.density = float(scene.density)
+    p.stab_iters = stab
+    p.solver_iters = solver
+    p.mu_s = mu_s
+    p.mu_k = mu_k
+    p.boundary_radius = 0.01
+    p.contact_radius_min = 0.01
+    p.sleep_speed = 0.01
+    p.damp_speed = 1.0
+    p.damping = 0.99
+    p.adaptive_damping = 0.98
+    p.skin = skin
+    p.own_only = own_only
+    return p
+
+
+def grid_build(pos: np.ndarray, cell_length, cells):
+    lib = load()
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    n = pos.shape[0]
+    C_ = cells[0] * cells[1] * cells[2]
+    keys = np.empty(n, np.int32)
+    perm = np.empty(n, np.int32)
+    cs = np.empty(C_ + 2, np.int32)
+    lib.oracle_grid_build(pos.reshape(-1), n, float(cell_length), *cells, keys, perm, cs)
+    return keys, perm, cs
+
+
+def sorted_boundary(bpos: np.ndarray, cell_length, cells):
+    """Boundary positions sorted by cell (stable) and its cell table."""
+    keys, perm, cs = grid_build(bpos, cell_length, cells)
+    return np.ascontiguousarray(bpos[perm]), cs
+
+
+def plain_steps_f32(pos, vel, mass, bpos_sorted, bcs, params, nsteps, adaptive_last=None):
+    lib = load()
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    vel = np.ascontiguousarray(vel, np.float32).copy()
+    mass = np.ascontiguousarray(mass, np.float32).copy()
+    n = pos.shape[0]
+    ms = np.empty(n, np.float32)
+    p2c = np.empty(n, np.int32)
+    adl = None if adaptive_last is None else np.ascontiguousarray(adaptive_last, np.int32).copy()
+    rc = lib.oracle_plain_steps_f32(
+        n, pos.reshape(-1), vel.reshape(-1), mass, bpos_sorted.shape[0],
+        np.ascontiguousarray(bpos_sorted, np.float32).reshape(-1), bcs, C.byref(params), nsteps,
+        ms.ctypes.data, p2c.ctypes.data, None if adl is None else adl.ctypes.data)
+    assert rc == 0
+    return pos, vel, mass, ms, p2c, adl
+
+
+def plain_steps_f64(pos, vel, mass, bpos_sorted, bcs, params, nsteps):
+    """Independent FP64 run; returns (pos, vel, mass, ids) with ids the
+    original index of each sorted particle."""
+    lib = load()
+    pos = np.ascontiguousarray(pos, np.float64).copy()
+    vel = np.ascontiguousarray(vel, np.float64).copy()
+    mass = np.ascontiguousarray(mass, np.float64).copy()
+    n = pos.shape[0]
+    ids = np.arange(n, dtype=np.int32)
+    rc = lib.oracle_plain_steps_f64(
+        n, pos.reshape(-1), vel.reshape(-1), mass, bpos_sorted.shape[0],
+        np.ascontiguousarray(bpos_sorted, np.float64).reshape(-1), bcs, C.byref(params), nsteps,
+        None, ids.ctypes.data)
+    assert rc == 0
+    return pos, vel, mass, ids
+
+
+def shadow_steps(pos, vel, mass, bpos_sorted, bcs, params, nsteps):
+    """Lockstep FP32 + decision-shadowed FP64 run.
+    Returns (pos32, vel32, pos64, vel64) in the same (sorted) order."""
+    lib = load()
+    p32 = np.ascontiguousarray(pos, np.float32).copy()
+    v32 = np.ascontiguousarray(vel, np.float32).copy()
+    m32 = np.ascontiguousarray(mass, np.float32).copy()
+    p64 = p32.astype(np.float64)
+    v64 = v32.astype(np.float64)
+    m64 = m32.astype(np.float64)
+    b32 = np.ascontiguousarray(bpos_sorted, np.float32)
+    b64 = b32.astype(np.float64)
+    rc = lib.oracle_shadow_steps(p32.shape[0], p32.reshape(-1), v32.reshape(-1), m32,
+                                 p64.reshape(-1), v64.reshape(-1), m64, b32.shape[0],
+                                 b32.reshape(-1), b64.reshape(-1), bcs, C.byref(params), nsteps,
+                                 None)
+    assert rc == 0
+    return p32, v32, p64, v64

@@ -1,0 +1,55 @@
+"""The C-ABI library loads without a GPU and exports every entry point that
+include/granular_gpu.h declares (no compute calls)."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    txt = (ROOT / "include" / "granular_gpu.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    names = header_functions()
+    for must in ("gg_grid_build", "gg_step", "gg_contact_sweep", "gg_predict", "gg_finalize",
+                 "gg_boundary_build", "gg_workspace_bytes"):
+        assert must in names
+
+
+def test_library_exports_all_header_symbols():
+    from granular_gpu_amd import _lib
+    lib = _lib.load()
+    missing = [n for n in header_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], check=True,
+                         capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (gg_[a-z0-9_]+)", out))
+    assert set(header_functions()) <= exported
+
+
+def test_binding_matches_header():
+    from granular_gpu_amd import _lib
+    assert set(_lib.declared_symbols()) <= set(header_functions())
+
+
+def test_host_only_calls():
+    from granular_gpu_amd import _lib
+    lib = _lib.load()
+    assert lib.gg_version().startswith(b"granular_gpu_amd")
+    p = _lib.default_params()
+    assert p.stab_iters == 3 and p.solver_iters == 5
+    assert abs(p.density - 238732.4146) < 0.1
+    g = _lib.gg_grid(0.0404, 25, 25, 25)
+    assert lib.gg_workspace_bytes(32768, 14408, g, 13, 48, 16) > 0
+    assert lib.gg_workspace_bytes(-1, 0, g, 13, 48, 16) == 0
+    assert lib.gg_status_string(-2) == b"capacity exceeded"
+    # invalid workspace is rejected before any device work
+    ws = _lib.gg_workspace()
+    assert lib.gg_workspace_init(C.byref(ws), None) == _lib.GG_ERR_INVALID

@@ -1,0 +1,209 @@
+"""GPU parity of the plain step: the HIP path through the C-ABI must equal
+the FP32 oracle bit for bit (cell ids, sort order, neighbour lists,
+positions, velocities) and stay within tolerance of the decision-shadowed
+FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from granular_gpu_amd import scenes
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _sim(scene, **kw):
+    from granular_gpu_amd.device import PlainSim
+    return PlainSim(scene, **kw)
+
+
+def _oracle_steps(scene, steps, stab, solver, mu_s=0.0, mu_k=0.0):
+    p = po.params_from(scene, stab, solver, mu_s, mu_k)
+    bpos, bcs = po.sorted_boundary(scene.boundary, scene.cell_length, scene.cells)
+    return po.plain_steps_f32(scene.pos, scene.vel, scene.mass, bpos, bcs, p, steps)
+
+
+def test_device_is_gfx950(gpu):
+    assert gpu.gg_device_check() == 0
+
+
+@pytest.mark.parametrize("case", ["lattice", "random_outside", "dense_cells"])
+def test_grid_build_bitexact(gpu, case):
+    import torch
+    _log(f"grid_build case={case}")
+    s = scenes.plain_box(20, 1.0)
+    rng = np.random.default_rng(7)
+    if case == "random_outside":
+        pos = rng.uniform(-0.1, 1.1, size=(30000, 3)).astype(np.float32)
+        pos[::97, 0] = np.nan
+        pos[5::101, 1] = np.inf
+    elif case == "dense_cells":
+        # one cell with 5000 (rank-sort path), cells with 40..2000 (LDS path)
+        a = np.full((5000, 3), 0.5, np.float32) + rng.uniform(0, 0.005, (5000, 3)).astype(
+            np.float32)
+        b = np.full((2000, 3), 0.1, np.float32) + rng.uniform(0, 0.005, (2000, 3)).astype(
+            np.float32)
+        c = rng.uniform(0.2, 0.3, size=(3000, 3)).astype(np.float32)
+        pos = np.concatenate([a, b, c])[rng.permutation(10000)]
+    else:
+        pos = s.pos
+    s.pos = pos
+    s.vel = rng.normal(size=pos.shape).astype(np.float32)
+    s.mass = rng.uniform(0.5, 4, size=pos.shape[0]).astype(np.float32)
+    sim = _sim(s)
+    perm = torch.zeros(pos.shape[0], dtype=torch.int32, device="cuda")
+    sim.grid_build(perm)
+    torch.cuda.synchronize()
+    keys, operm, cs = po.grid_build(pos, s.cell_length, s.cells)
+    assert np.array_equal(perm.cpu().numpy(), operm)
+    assert np.array_equal(sim.p2c[: s.n].cpu().numpy(), keys)
+    assert np.array_equal(sim.cs.cpu().numpy(), cs)
+    gp, gv, gm = sim.host()
+    assert np.array_equal(gp, pos[operm], equal_nan=True)
+    assert np.array_equal(gv, s.vel[operm])
+    assert np.array_equal(gm, s.mass[operm])
+
+
+@pytest.mark.parametrize("steps,stab,solver,fric,lists", [
+    (1, 3, 5, (0.0, 0.0), True),
+    (1, 3, 10, (0.5, 0.4), True),
+    (10, 3, 10, (0.0, 0.0), True),
+    (10, 3, 10, (0.5, 0.4), True),
+    (10, 3, 10, (0.5, 0.4), False),
+    (3, 0, 4, (0.0, 0.0), True),
+    (3, 2, 0, (0.0, 0.0), True),
+])
+def test_step_bitexact_vs_fp32_oracle(gpu, steps, stab, solver, fric, lists):
+    _log(f"step steps={steps} S={stab} I={solver} fric={fric} lists={lists}")
+    s = scenes.plain_box(32, 1.0)  # 32,768 particles, 25^3 cells, 14,408 walls
+    sim = _sim(s, stab=stab, solver=solver, mu_s=fric[0], mu_k=fric[1],
+               skin=0.005 if lists else 0.0)
+    sim.step(steps)
+    gp, gv, gm = sim.host()
+    op, ov, om, oms, op2c, _ = _oracle_steps(s, steps, stab, solver, *fric)
+    assert np.array_equal(sim.p2c[: s.n].cpu().numpy(), op2c)
+    bad = np.nonzero(np.any(gp != op, axis=1))[0]
+    assert bad.size == 0, (bad[:10], gp[bad[:3]], op[bad[:3]])
+    assert np.array_equal(gv, ov)
+    assert np.array_equal(gm, om)
+    assert np.array_equal(sim.scaled_mass[: s.n].cpu().numpy(), oms)
+
+
+def test_step_within_tolerance_of_fp64(gpu):
+    _log("fp64 shadow tolerance, C1 + friction")
+    s = scenes.plain_box(32, 1.0)
+    p = po.params_from(s, 3, 10, 0.5, 0.4)
+    bpos, bcs = po.sorted_boundary(s.boundary, s.cell_length, s.cells)
+    for steps, tol in ((1, 1e-5 * 0.01), (10, 1e-3 * 0.01)):
+        sim = _sim(s, stab=3, solver=10, mu_s=0.5, mu_k=0.4)
+        sim.step(steps)
+        gp = sim.host()[0]
+        _, _, p64, _ = po.shadow_steps(s.pos, s.vel, s.mass, bpos, bcs, p, steps)
+        assert np.abs(gp.astype(np.float64) - p64).max() <= tol
+
+
+def test_neighbour_lists_bitexact(gpu):
+    import torch
+    _log("neighbour lists")
+    s = scenes.plain_box(24, 1.0)
+    gmax, bmax = 48, 16
+    sim = _sim(s, gmax=gmax, bmax=bmax)
+    sim.grid_build()
+    sim.predict()
+    torch.cuda.synchronize()
+    n = s.n
+    cap = sim.cap + 1
+    x0 = sim.view(0, n)
+    off = sim.view(1, n)
+    meta = sim.view(2, n, dtype=torch.int32)
+    gl = sim.view(3, gmax * cap, dtype=torch.int32, width=1).reshape(gmax, cap)[:, :n]
+    bl = sim.view(4, bmax * cap, dtype=torch.int32, width=1).reshape(bmax, cap)[:, :n]
+    x0, off, meta = x0.cpu().numpy(), off.cpu().numpy(), meta.cpu().numpy()
+    gl, bl = gl.cpu().numpy(), bl.cpu().numpy()
+    p = po.params_from(s, skin=0.005)
+    bpos, bcs = po.sorted_boundary(s.boundary, s.cell_length, s.cells)
+    cs = sim.cs.cpu().numpy()
+    ogc = np.empty(n, np.int32); ogl = np.empty(n * gmax, np.int32)
+    obc = np.empty(n, np.int32); obl = np.empty(n * bmax, np.int32)
+    obase = np.empty(n, np.int32)
+    po.load().oracle_nlist_f32(n, np.ascontiguousarray(x0).reshape(-1),
+                               np.ascontiguousarray(off).reshape(-1), bpos.shape[0],
+                               bpos.reshape(-1), cs, bcs, p, gmax, bmax, ogc, ogl, obc, obl,
+                               obase)
+    assert np.array_equal(meta[:, 0], ogc)
+    assert np.array_equal(meta[:, 1], obc)
+    assert ogc.max() <= gmax and obc.max() <= bmax
+    ogl = ogl.reshape(n, gmax)
+    obl = obl.reshape(n, bmax)
+    for i in range(n):
+        assert np.array_equal(gl[: ogc[i], i], ogl[i, : ogc[i]])
+        assert np.array_equal(bl[: obc[i], i], obl[i, : obc[i]])
+
+
+def test_run_twice_bitwise_identical(gpu):
+    _log("determinism")
+    s = scenes.plain_box(32, 1.0)
+    a = _sim(s, stab=3, solver=10, mu_s=0.5, mu_k=0.4)
+    b = _sim(s, stab=3, solver=10, mu_s=0.5, mu_k=0.4)
+    a.step(20)
+    b.step(20)
+    for x, y in zip(a.host(), b.host()):
+        assert np.array_equal(x, y)
+
+
+def test_edge_sizes(gpu):
+    import ctypes as C
+    from granular_gpu_amd import _lib
+    _log("edge sizes")
+    s = scenes.plain_box(2, 1.0)
+    s.pos = s.pos[:1].copy()
+    s.vel = np.array([[0.3, 0.0, 0.0]], np.float32)
+    s.mass = s.mass[:1].copy()
+    sim = _sim(s)
+    sim.step(2)
+    op = _oracle_steps(s, 2, 3, 5)[0]
+    assert np.array_equal(sim.host()[0], op)
+    # empty set
+    sim.n = 0
+    sim._refresh_set()
+    sim.step(1)
+    # capacity overflow is reported, not executed
+    sim.n = sim.cap + 1
+    sim._refresh_set()
+    with pytest.raises(_lib.GGError) as e:
+        sim.step(1)
+    assert e.value.status == _lib.GG_ERR_CAPACITY
+    sim.n = 1
+    sim._refresh_set()
+    # too many sweeps for the workspace
+    sim.params.solver_iters = 50
+    with pytest.raises(_lib.GGError):
+        sim.step(1)
+
+
+def test_m1_full_size_properties(gpu):
+    import torch
+    _log("M1 1M particles, 3 steps")
+    s = scenes.plain_box(100, 2.4)
+    sim = _sim(s, stab=3, solver=10, mu_s=0.5, mu_k=0.4)
+    sim.step(3)
+    torch.cuda.synchronize()
+    keys = sim.p2c[: s.n].cpu().numpy()
+    cs = sim.cs.cpu().numpy()
+    pos, vel, mass = sim.host()
+    assert np.all(np.diff(keys) >= 0)
+    counts = np.bincount(keys, minlength=60 ** 3 + 1)
+    assert np.array_equal(cs, np.concatenate([[0], np.cumsum(counts)]))
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+    assert np.array_equal(np.sort(mass), np.ones(s.n, np.float32))
+    # centre of mass is still in free fall (no floor contact yet): y drop of
+    # dt^2 g k(k+1)/2 within jitter-contact noise
+    dt, g = float(s.dt), float(s.gravity[1])
+    drop = dt * dt * g * 3 * 4 / 2
+    assert abs(pos[:, 1].mean() - (s.pos[:, 1].mean() + drop)) < 1e-5
