@@ -48,7 +48,20 @@ def main():
     ap.add_argument("--cpu-baseline-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stage-timing", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 512 MiB device-to-device copy (calibrates HBM counters)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        a = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}), flush=True)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -88,6 +101,7 @@ def main():
     K = S + I
     if not args.no_stage_timing:
         lib.gg_profiler_enable(args.steps * (K + 4) + 16)
+    # solver sweeps are the dominant kernel: 10 of 13 sweeps per step
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.step(args.steps)
@@ -119,7 +133,7 @@ def main():
                     traffic = json.loads(pmc.read_text()).get("hbm_bytes_per_launch")
                 except Exception:
                     traffic = None
-            roofline = {"kernel": "k_sweep (contact sweep)", "bound": "hbm",
+            roofline = {"kernel": "k_sweep<true,false> (solver contact sweep)", "bound": "hbm",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "algorithmic_bytes_per_launch": per_launch_bytes,

@@ -20,7 +20,7 @@ GG_ERR_HIP = -3
 GG_ERR_WORKSPACE = -4
 GG_ERR_NO_DEVICE = -5
 GG_FLAG_NO_NLIST = 1
-STAGES = ["sort", "predict", "nlist", "sweep", "finalize"]
+STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep"]
 
 
 class gg_grid(C.Structure):

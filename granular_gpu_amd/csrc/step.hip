@@ -412,7 +412,8 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   a.use_lists = lists_on(p, ws) ? 1 : 0;
   a.remap = (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1;
   const int nb = blocks_for(n);
-  prof_begin(GG_STAGE_SWEEP, st);
+  const int stage = k < S ? GG_STAGE_STAB : GG_STAGE_SWEEP;
+  prof_begin(stage, st);
   int rc;
   if (last_stab)
     rc = launch_sweep<false, true>(a, nb, st);
@@ -421,7 +422,7 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   else
     rc = launch_sweep<false, false>(a, nb, st);
   if (rc != GG_OK) return rc;
-  prof_end(GG_STAGE_SWEEP, st);
+  prof_end(stage, st);
   return GG_OK;
 }
 

@@ -186,9 +186,10 @@ enum {
   GG_STAGE_SORT = 0,     /* hash + counting sort + cell table */
   GG_STAGE_PREDICT = 1,  /* permute + predict */
   GG_STAGE_NLIST = 2,    /* neighbour candidate lists */
-  GG_STAGE_SWEEP = 3,    /* one contact sweep (stabilisation or solver) */
+  GG_STAGE_SWEEP = 3,    /* one solver contact sweep */
   GG_STAGE_FINALIZE = 4, /* final update */
-  GG_STAGE_COUNT = 5
+  GG_STAGE_STAB = 5,     /* one stabilisation contact sweep */
+  GG_STAGE_COUNT = 6
 };
 int gg_profiler_enable(int max_records); /* 0 disables */
 int gg_profiler_collect(double *total_ms, int *count, int nstages);
