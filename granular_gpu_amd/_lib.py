@@ -20,7 +20,7 @@ GG_ERR_HIP = -3
 GG_ERR_WORKSPACE = -4
 GG_ERR_NO_DEVICE = -5
 GG_FLAG_NO_NLIST = 1
-STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep"]
+STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep", "adaptive"]
 
 
 class gg_grid(C.Structure):
@@ -82,6 +82,16 @@ _PROTOS = {
                                    P(gg_workspace), VP]),
     "gg_finalize": (C.c_int, [P(gg_particles), P(gg_params), P(gg_workspace), VP]),
     "gg_workspace_view": (VP, [P(gg_workspace), C.c_int]),
+    "gg_surface": (C.c_int, [P(gg_particles), VP, P(gg_params), P(gg_workspace), VP]),
+    "gg_surface_distance": (C.c_int, [P(gg_particles), VP, P(gg_workspace), VP]),
+    "gg_merge_mark": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, C.c_float, P(gg_workspace),
+                                VP]),
+    "gg_merge_blend": (C.c_int, [P(gg_particles), P(gg_adaptive), C.c_int, P(gg_workspace), VP]),
+    "gg_split_compact": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, P(gg_particles), VP,
+                                   P(gg_params), P(gg_workspace), VP, P(C.c_int)]),
+    "gg_adaptive_sampling": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, P(gg_particles), VP,
+                                       P(gg_params), C.c_float, C.c_int, C.c_int,
+                                       P(gg_workspace), VP, P(C.c_int)]),
     "gg_profiler_enable": (C.c_int, [C.c_int]),
     "gg_profiler_collect": (C.c_int, [P(C.c_double), P(C.c_int), C.c_int]),
 }

@@ -22,6 +22,8 @@ constexpr float kPiF = 3.14159265358979323846f;
 // the skin.
 constexpr float kListValidFrac = 0.9f;
 constexpr int kFallbackSlots = 62;  // ctr[1 + k % 62] = full-stencil count of sweep k
+// ctr[63..] are not used by the sweeps; the adaptive passes use ctr2 (own region)
+constexpr int kMergeRounds = 12;    // parallel claim-resolution rounds before the finisher
 
 // Byte offsets of every buffer inside the caller's workspace.  Every region
 // starts on a 256-byte boundary.
@@ -34,6 +36,7 @@ struct Layout {
   size_t part;     // int[parts+1]   scan partials
   size_t big;      // int[C+1]       list of cells needing the big sort
   size_t ctr;      // int[64]        device counters ([0] = big-cell count)
+  size_t ctr2;     // int[64]        adaptive counters
   size_t dirty;    // int[C+1]       cells near a particle past half the skin
   size_t x0;       // float4[cap]    start-of-step position + radius
   size_t off0;     // float4[cap]    offset ping
@@ -50,6 +53,20 @@ struct Layout {
   size_t v3tmp0;   // float3[cmax]
   size_t v3tmp1;   // float3[cmax]
   size_t bnd4;     // float4[bcap]   packed sorted boundary
+  // adaptive merge/split
+  size_t cover;    // float4[cap]    cover vectors
+  size_t cand;     // int[cap]       merge candidate
+  size_t dens;     // float[cap]     local density
+  size_t mstate;   // int[cap]       merge claim state
+  size_t owner;    // int[cap]       lowest successful claimer per target
+  size_t und0;     // int[cap]       lowest undecided claimer per target (ping)
+  size_t und1;     // int[cap]       (pong)
+  size_t flags;    // int[cap]       generic predicate / position scratch
+  size_t bsum;     // int[blocks+1]  per-block counts for ordered compaction
+  size_t bsum2;    // int[blocks+1]
+  size_t child;    // float4[cap]    split child positions
+  size_t itmp2;    // int[cmax]
+  size_t itmp3;    // int[cmax]
   size_t total;
 };
 
@@ -65,7 +82,8 @@ inline Layout make_layout(int capacity, int bcap, gg_grid g, int max_sweeps,
   const size_t C1 = (size_t)grid_cells(g) + 1;
   const size_t cmax = (size_t)(capacity > bcap ? capacity : bcap) + 1;
   const size_t cap = (size_t)capacity + 1;
-  const size_t parts = (C1 + kScanTile - 1) / kScanTile + 1;
+  const size_t nblk0 = (cap + kBlock - 1) / kBlock + 2;
+  const size_t parts = ((C1 > nblk0 ? C1 : nblk0) + kScanTile - 1) / kScanTile + 1;
   size_t o = 0;
   auto take = [&](size_t bytes) {
     size_t at = o;
@@ -80,6 +98,7 @@ inline Layout make_layout(int capacity, int bcap, gg_grid g, int max_sweeps,
   L.part = take(parts * 4);
   L.big = take(C1 * 4);
   L.ctr = take(64 * 4);
+  L.ctr2 = take(64 * 4);
   L.dirty = take(C1 * 4);
   L.x0 = take(cap * 16);
   L.off0 = take(cap * 16);
@@ -96,6 +115,20 @@ inline Layout make_layout(int capacity, int bcap, gg_grid g, int max_sweeps,
   L.v3tmp0 = take(cmax * 12);
   L.v3tmp1 = take(cmax * 12);
   L.bnd4 = take(((size_t)bcap + 1) * 16);
+  const size_t nblk = (cap + kBlock - 1) / kBlock + 2;
+  L.cover = take(cap * 16);
+  L.cand = take(cap * 4);
+  L.dens = take(cap * 4);
+  L.mstate = take(cap * 4);
+  L.owner = take(cap * 4);
+  L.und0 = take(cap * 4);
+  L.und1 = take(cap * 4);
+  L.flags = take(cap * 4);
+  L.bsum = take(nblk * 4);
+  L.bsum2 = take(nblk * 4);
+  L.child = take(cap * 16);
+  L.itmp2 = take(cmax * 4);
+  L.itmp3 = take(cmax * 4);
   L.total = o;
   return L;
 }

@@ -335,6 +335,24 @@ int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *w
   return GG_OK;
 }
 
+// out[k] = sum of in[0..k) for k in [0, m], in[] cleared.  m <= kScanTile *
+// kScanMaxParts (the workspace's partials array holds the tile sums).
+int scan_exclusive(int *in, int m, int *out, const gg_workspace *ws, hipStream_t st) {
+  if (m <= 0) return GG_OK;
+  if ((long long)m > (long long)kScanTile * kScanMaxParts) return GG_ERR_CAPACITY;
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  int *part = at<int>(ws, L.part);
+  const int np = (m + kScanTile - 1) / kScanTile;
+  k_scan_reduce<<<np, kBlock, 0, st>>>(in, m, part);
+  GG_LAUNCH_CHECK();
+  k_scan_parts<<<1, 1024, 0, st>>>(part, np);
+  GG_LAUNCH_CHECK();
+  k_scan_apply<<<np, kBlock, 0, st>>>(in, m, part, out);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+
 static int gather_inplace(void *arr, int words, const int *perm, int n, void *tmp,
                           hipStream_t st) {
   if (!arr || n <= 0) return GG_OK;

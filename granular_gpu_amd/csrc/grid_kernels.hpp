@@ -10,4 +10,6 @@ int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *w
 // sort_by_cell + in-place permutation of the set's payload arrays.
 int grid_build_general(gg_particles *set, gg_adaptive *ad, int *cell_start, int *perm_out,
                        const gg_workspace *ws, hipStream_t st, int capacity_limit);
+// Exclusive scan of m ints: out[0..m] (out[m] = total); in[] is cleared.
+int scan_exclusive(int *in, int m, int *out, const gg_workspace *ws, hipStream_t st);
 }  // namespace gg

@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, load
+from ._lib import check, load  # noqa: F401
 
 
 def _ptr(t: torch.Tensor | None):
@@ -140,3 +140,68 @@ class PlainSim:
         n = self.n
         return (self.pos[:n].cpu().numpy().copy(), self.vel[:n].cpu().numpy().copy(),
                 self.mass[:n].cpu().numpy().copy())
+
+
+class AdaptiveSim(PlainSim):
+    """Scene with adaptive merge/split (reference is_adaptive == 1): the plain
+    step, then surface flags, distance to the surface and
+    Solver::adaptive_sampling (GranularSystem.cu:420-432)."""
+
+    def __init__(self, scene, capacity=None, max_mass=4.0, blend=40, merge_first=False, **kw):
+        cap = capacity or 2 * scene.n
+        super().__init__(scene, capacity=cap, **kw)
+        dev = self.dev
+        i32, f32 = torch.int32, torch.float32
+        self.remove = torch.zeros(cap, dtype=i32, device=dev)
+        self.merge = torch.zeros(cap, dtype=f32, device=dev)
+        self.mcount = torch.zeros(cap, dtype=i32, device=dev)
+        self.mv = torch.zeros((cap, 3), dtype=f32, device=dev)
+        self.surface = torch.zeros(cap, dtype=i32, device=dev)
+        self.sd = torch.zeros(cap, dtype=f32, device=dev)
+        self.num_surface = torch.zeros(cap, dtype=i32, device=dev)
+        self.max_mass = max_mass
+        self.blend = blend
+        self.merge_first = merge_first
+        self.calls = 0
+        self.ad = _lib.gg_adaptive(_ptr(self.remove), _ptr(self.merge), _ptr(self.mv),
+                                   _ptr(self.mcount))
+        self.stats = []
+        self._refresh_set()
+
+    def _refresh_set(self):
+        super()._refresh_set()
+        if hasattr(self, "surface"):
+            self.set.surface = _ptr(self.surface)
+            self.set.surface_distance = _ptr(self.sd)
+            self.set.num_surface = _ptr(self.num_surface)
+
+    def step(self, nsteps=1):
+        lib = self.lib
+        st = (C.c_int * 3)()
+        for _ in range(nsteps):
+            s = _stream()
+            check("gg_step", lib.gg_step(C.byref(self.set), C.byref(self.ad), _ptr(self.cs),
+                                         C.byref(self.bset), _ptr(self.bcs),
+                                         C.byref(self.params), C.byref(self.ws), s))
+            check("gg_surface", lib.gg_surface(C.byref(self.set), _ptr(self.cs),
+                                               C.byref(self.params), C.byref(self.ws), s))
+            check("gg_surface_distance", lib.gg_surface_distance(C.byref(self.set), _ptr(self.cs),
+                                                                 C.byref(self.ws), s))
+            do_merge = 1 if (self.calls > 0 or self.merge_first) else 0
+            n_before = self.set.n
+            check("gg_adaptive_sampling",
+                  lib.gg_adaptive_sampling(C.byref(self.set), C.byref(self.ad), _ptr(self.cs),
+                                           C.byref(self.bset), _ptr(self.bcs),
+                                           C.byref(self.params), self.max_mass, do_merge,
+                                           self.blend, C.byref(self.ws), s, st))
+            self.n = self.set.n
+            self.calls += 1
+            self.stats.append((st[0], st[1], n_before))
+
+    def host_all(self):
+        torch.cuda.synchronize()
+        n = self.n
+        g = lambda t: t[:n].cpu().numpy().copy()
+        return dict(pos=g(self.pos), vel=g(self.vel), mass=g(self.mass), adl=g(self.adl),
+                    remove=g(self.remove), merge=g(self.merge), mcount=g(self.mcount),
+                    surface=g(self.surface), sd=g(self.sd))

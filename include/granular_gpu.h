@@ -158,6 +158,44 @@ int gg_step(gg_particles *set, gg_adaptive *ad, int *cell_start,
             const gg_params *params, const gg_workspace *ws,
             hipStream_t stream);
 
+/* ---- adaptive merge/split ----------------------------------------------- */
+/* Surface flags into set->surface (K5 find_cover_vector + K6 find_surface,
+ * GranularSystem.cu:506-637).  Uses the cell table of the start of the step
+ * with the current positions, as the reference does. */
+int gg_surface(gg_particles *set, const int *cell_start, const gg_params *params,
+               const gg_workspace *ws, hipStream_t stream);
+/* Squared distance to the nearest surface particle into
+ * set->surface_distance (K7 compute_distance_to_surface,
+ * GranularSystem.cu:639-699). */
+int gg_surface_distance(gg_particles *set, const int *cell_start,
+                        const gg_workspace *ws, hipStream_t stream);
+/* Merge pairing (K8 merge_mark_gpu, Solver.cu:526-653): candidates from the
+ * pre-kernel snapshot of remove[], claims resolved as if threads ran in
+ * ascending index order.  Writes remove, merge, merge_velocity. */
+int gg_merge_mark(gg_particles *set, gg_adaptive *ad, const int *cell_start,
+                  float max_mass, const gg_workspace *ws, hipStream_t stream);
+/* Mass/velocity blend over blend_steps steps (K9 merge_count_gpu,
+ * Solver.cu:658-699; the reference uses 40). */
+int gg_merge_blend(gg_particles *set, gg_adaptive *ad, int blend_steps,
+                   const gg_workspace *ws, hipStream_t stream);
+/* Split (K10/K11, Solver.cu:708-843) + compaction of pos, vel, mass,
+ * adaptive_last_step, remove, merge, merge_count (survivors keep their order)
+ * + append of the children in ascending parent order (Solver.cu:1037-1221).
+ * At most capacity - survivors children are made.  Synchronises the stream
+ * and writes the new count to set->n; stats (optional, 3 ints) = removed,
+ * split, candidates refused for capacity. */
+int gg_split_compact(gg_particles *set, gg_adaptive *ad, const int *cell_start,
+                     const gg_particles *boundary, const int *bcell_start,
+                     const gg_params *params, const gg_workspace *ws,
+                     hipStream_t stream, int *stats);
+/* Solver::adaptive_sampling (Solver.cu:856-1242): merge mark (if do_merge),
+ * blend, split + compaction. */
+int gg_adaptive_sampling(gg_particles *set, gg_adaptive *ad, const int *cell_start,
+                         const gg_particles *boundary, const int *bcell_start,
+                         const gg_params *params, float max_mass, int do_merge,
+                         int blend_steps, const gg_workspace *ws,
+                         hipStream_t stream, int *stats);
+
 /* Stage entry points of gg_step, in order (for stage-level checking).
  * gg_predict: T7-T9 (Solver.cu:125-191) on the sorted set; builds the
  *   neighbour lists unless GG_FLAG_NO_NLIST.
@@ -189,7 +227,8 @@ enum {
   GG_STAGE_SWEEP = 3,    /* one solver contact sweep */
   GG_STAGE_FINALIZE = 4, /* final update */
   GG_STAGE_STAB = 5,     /* one stabilisation contact sweep */
-  GG_STAGE_COUNT = 6
+  GG_STAGE_ADAPTIVE = 6, /* surface / distance / merge / split passes */
+  GG_STAGE_COUNT = 7
 };
 int gg_profiler_enable(int max_records); /* 0 disables */
 int gg_profiler_collect(double *total_ms, int *count, int nstages);

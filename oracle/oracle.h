@@ -98,6 +98,43 @@ int oracle_nlist_f32(int n, const float *base4, const float *off4, int nb,
                      int bmax, int *gcount, int *glist, int *bcount,
                      int *blist, int *base_cell);
 
+/* ---- adaptive merge/split (GranularSystem.cu:506-699, Solver.cu:526-1242) */
+/* Surface flags (K5+K6): cover vector and surface = n <= 5 or no neighbour
+ * inside the pi/3 cone around the cover direction (tested as dot >= 0.5). */
+int oracle_surface(int n, const float *pos, const float *mass, const int *cs,
+                   const oracle_params *p, float *cover, int *surface);
+/* K7: squared distance to the nearest surface particle of the stencil.    */
+int oracle_surface_distance(int n, const float *pos, const int *surface,
+                            const int *cs, const oracle_params *p, float *sd);
+/* K8 with the pairing resolved as if threads ran in ascending index:
+ * candidates from the pre-kernel snapshot of remove[]; i claims itself
+ * unless already claimed, then its candidate unless already claimed.
+ * cand/dens (optional) receive each particle's candidate and density.     */
+int oracle_merge_mark(int n, const float *pos, const float *mass,
+                      const float *vel, const int *surface, const float *sd,
+                      int *remove, float *merge, float *merge_velocity,
+                      const int *cs, const oracle_params *p, float max_mass,
+                      int *cand, float *dens);
+/* K9: 40-step mass/velocity blend.                                         */
+int oracle_merge_blend(int n, float *merge, float *mass, float *vel,
+                       int *merge_count, int *remove,
+                       const float *merge_velocity, int blend, int *adl);
+/* K10/K11 + compaction + append; returns the new particle count.          */
+int oracle_split_compact(int n, int cap, float *pos, float *vel, float *mass,
+                         int *adl, int *remove, float *merge,
+                         int *merge_count, const int *surface,
+                         const float *bpos, const int *cs, const int *bcs,
+                         const oracle_params *p, int *stats);
+/* Whole adaptive steps (plain step + the passes above).  stats[3k..3k+2] =
+ * (removed, split, count before) of step k.                               */
+int oracle_adaptive_steps(int *n, int cap, float *pos, float *vel, float *mass,
+                          int *adl, int *remove, float *merge,
+                          int *merge_count, float *merge_velocity,
+                          int *surface, float *sd, int nb, const float *bpos,
+                          const int *bcell_start, const oracle_params *p,
+                          int nsteps, int merge_first, float max_mass,
+                          int *stats);
+
 /* Radius helpers (three reference variants), exposed for unit tests.     */
 float oracle_radius_contact(float m, float density);
 float oracle_radius_surface(float m, float density);

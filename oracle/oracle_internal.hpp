@@ -1,0 +1,21 @@
+// Declarations shared between the oracle's translation units -- TEST
+// INFRASTRUCTURE ONLY.
+#pragma once
+#include "oracle_common.hpp"
+
+namespace orc {
+void surface(int n, const V3<float> *pos, const float *mass, const int *cs, const Grid &g,
+             float density, V3<float> *cover, int *surf);
+void surface_distance(int n, const V3<float> *pos, const int *surf, const int *cs, const Grid &g,
+                      float *sd);
+void merge_mark(int n, const V3<float> *pos, const float *mass, const V3<float> *vel,
+                const int *surf, const float *sd, int *remove, float *merge, V3<float> *mv,
+                const int *cs, const Grid &g, float max_mass, int *cand_out, float *dens_out);
+void merge_blend(int n, float *merge, float *mass, V3<float> *vel, int *mcount, int *remove,
+                 const V3<float> *mv, int blend, int *adl);
+}  // namespace orc
+
+extern "C" int oracle_split_compact(int n, int cap, float *pos, float *vel, float *mass,
+                                    int *adl, int *remove, float *merge, int *mcount,
+                                    const int *surf, const float *bpos, const int *cs,
+                                    const int *bcs, const oracle_params *p, int *stats);

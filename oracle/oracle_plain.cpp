@@ -4,7 +4,7 @@
 // -ffp-contract=off and without -ffast-math: every FP32 operation below is an
 // individually rounded IEEE operation, in the order written.
 #include "oracle.h"
-#include "oracle_common.hpp"
+#include "oracle_internal.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -699,4 +699,86 @@ extern "C" int oracle_shadow_steps(int n, float *pf, float *vf, float *mf,
   if (n < 0 || nb < 0 || !p) return -1;
   return shadow_steps(n, pf, vf, mf, pd, vd, md, nb, bpf, bpd, bcs, p, nsteps,
                       adaptive_last);
+}
+
+// Full adaptive steps: the plain step (the sort also permutes merge,
+// merge_count and remove, GranularSystem.cu:99-135; adaptive_last_step and
+// merge_velocity are not permuted), then set_surface_particles,
+// find_distance_to_surface and Solver::adaptive_sampling
+// (GranularSystem.cu:420-432, Solver.cu:856-1242) against the cell table of
+// the start of the step.  merge_first = 0 skips the merge mark of the first
+// step (the reference's t_merge_iter toggle, Solver.cu:910-922).
+extern "C" int oracle_adaptive_steps(int *n_io, int cap, float *pos_, float *vel_, float *mass,
+                                     int *adl, int *remove, float *merge, int *mcount,
+                                     float *mv_, int *surf, float *sd, int nb,
+                                     const float *bpos_, const int *bcs,
+                                     const oracle_params *p, int nsteps, int merge_first,
+                                     float max_mass, int *stats) {
+  Grid g{p->cell_length, p->cx, p->cy, p->cz};
+  const int C = g.cx * g.cy * g.cz;
+  std::vector<int> cs(C + 2), idx, keys;
+  V3<float> *pos = reinterpret_cast<V3<float> *>(pos_);
+  V3<float> *vel = reinterpret_cast<V3<float> *>(vel_);
+  V3<float> *mv = reinterpret_cast<V3<float> *>(mv_);
+  const int S = p->stab_iters, I = p->solver_iters;
+  (void)nb;
+  for (int step = 0; step < nsteps; ++step) {
+    const int n = *n_io;
+    Sim<float> s;
+    s.init(n, pos_, vel_, mass, bpos_, p);
+    sort_perm(pos, n, g, idx, keys, cs.data());
+    s.permute(idx);
+    {
+      std::vector<float> tm(n);
+      std::vector<int> ti(n), tr(n);
+      for (int k = 0; k < n; ++k) {
+        tm[k] = merge[idx[k]];
+        ti[k] = mcount[idx[k]];
+        tr[k] = remove[idx[k]];
+      }
+      for (int k = 0; k < n; ++k) {
+        merge[k] = tm[k];
+        mcount[k] = ti[k];
+        remove[k] = tr[k];
+      }
+    }
+    s.predict(p);
+    for (int k = 0; k < S; ++k) {
+#pragma omp parallel for schedule(dynamic, 256)
+      for (int i = 0; i < n; ++i)
+        s.b[i] = sweep_one<float, FREE>(i, s.pos, s.rad.data(), s.a.data(), s.bpos, cs.data(),
+                                        bcs, g, s.sp, false, nullptr, s.ims.data());
+      std::swap(s.a, s.b);
+    }
+    s.end_stabilisation(S);
+    for (int k = 0; k < I; ++k) {
+#pragma omp parallel for schedule(dynamic, 256)
+      for (int i = 0; i < n; ++i)
+        s.b[i] = sweep_one<float, FREE>(i, s.xb.data(), s.rad.data(), s.a.data(), s.bpos,
+                                        cs.data(), bcs, g, s.sp, true, nullptr, s.ims.data());
+      std::swap(s.a, s.b);
+    }
+    for (int i = 0; i < n; ++i) {
+      uint8_t bits = 0;
+      finalize_one<float, FREE>(i, s.pos, s.vel, s.xb.data(), s.a.data(), adl, s.sp, p->dt,
+                                bits);
+    }
+    for (int i = 0; i < n; ++i) adl[i] = 0;
+    std::vector<V3<float>> cover(n);
+    surface(n, pos, mass, cs.data(), g, p->density, cover.data(), surf);
+    surface_distance(n, pos, surf, cs.data(), g, sd);
+    if (step > 0 || merge_first)
+      merge_mark(n, pos, mass, vel, surf, sd, remove, merge, mv, cs.data(), g, max_mass,
+                 nullptr, nullptr);
+    merge_blend(n, merge, mass, vel, mcount, remove, mv, 40, adl);
+    int st[3] = {0, 0, 0};
+    *n_io = oracle_split_compact(n, cap, pos_, vel_, mass, adl, remove, merge, mcount, surf,
+                                 bpos_, cs.data(), bcs, p, st);
+    if (stats) {
+      stats[3 * step] = st[0];
+      stats[3 * step + 1] = st[1];
+      stats[3 * step + 2] = n;
+    }
+  }
+  return 0;
 }

@@ -57,6 +57,18 @@ def load() -> C.CDLL:
     lib.oracle_sweep_f32.argtypes = [C.c_int, fp, fp, fp, C.c_int, fp, ip, ip, P, C.c_int]
     lib.oracle_nlist_f32.argtypes = [C.c_int, fp, fp, C.c_int, fp, ip, ip, P, C.c_int, C.c_int,
                                      ip, ip, ip, ip, ip]
+    vp = C.c_void_p
+    lib.oracle_surface.argtypes = [C.c_int, fp, fp, ip, P, fp, ip]
+    lib.oracle_surface_distance.argtypes = [C.c_int, fp, ip, ip, P, fp]
+    lib.oracle_merge_mark.argtypes = [C.c_int, fp, fp, fp, ip, fp, ip, fp, fp, ip, P, C.c_float,
+                                      vp, vp]
+    lib.oracle_merge_blend.argtypes = [C.c_int, fp, fp, fp, ip, ip, fp, C.c_int, ip]
+    lib.oracle_split_compact.argtypes = [C.c_int, C.c_int, fp, fp, fp, ip, ip, fp, ip, ip, fp,
+                                         ip, ip, P, ip]
+    lib.oracle_split_compact.restype = C.c_int
+    lib.oracle_adaptive_steps.argtypes = [C.POINTER(C.c_int), C.c_int, fp, fp, fp, ip, ip, fp,
+                                          ip, fp, ip, fp, C.c_int, fp, ip, P, C.c_int, C.c_int,
+                                          C.c_float, ip]
     for nm in ("oracle_radius_contact", "oracle_radius_surface", "oracle_radius_split"):
         getattr(lib, nm).argtypes = [C.c_float, C.c_float]
         getattr(lib, nm).restype = C.c_float
@@ -162,3 +174,42 @@ def shadow_steps(pos, vel, mass, bpos_sorted, bcs, params, nsteps):
                                  None)
     assert rc == 0
     return p32, v32, p64, v64
+
+
+class AdaptiveState:
+    """Host copy of everything the adaptive path reads and writes."""
+
+    def __init__(self, pos, vel, mass, cap=None):
+        n = pos.shape[0]
+        self.cap = cap or n
+        cap = self.cap
+        self.n = n
+        self.pos = np.zeros((cap, 3), np.float32); self.pos[:n] = pos
+        self.vel = np.zeros((cap, 3), np.float32); self.vel[:n] = vel
+        self.mass = np.zeros(cap, np.float32); self.mass[:n] = mass
+        self.adl = np.zeros(cap, np.int32)
+        self.remove = np.zeros(cap, np.int32)
+        self.merge = np.zeros(cap, np.float32)
+        self.mcount = np.zeros(cap, np.int32)
+        self.mv = np.zeros((cap, 3), np.float32)
+        self.surface = np.zeros(cap, np.int32)
+        self.sd = np.zeros(cap, np.float32)
+
+    def copy(self):
+        import copy
+        return copy.deepcopy(self)
+
+
+def adaptive_steps(st: AdaptiveState, bpos_sorted, bcs, params, nsteps, merge_first=0,
+                   max_mass=4.0):
+    lib = load()
+    nn = C.c_int(st.n)
+    stats = np.zeros(3 * max(nsteps, 1), np.int32)
+    rc = lib.oracle_adaptive_steps(
+        C.byref(nn), st.cap, st.pos.reshape(-1), st.vel.reshape(-1), st.mass, st.adl, st.remove,
+        st.merge, st.mcount, st.mv.reshape(-1), st.surface, st.sd, bpos_sorted.shape[0],
+        np.ascontiguousarray(bpos_sorted, np.float32).reshape(-1), bcs, C.byref(params), nsteps,
+        merge_first, max_mass, stats)
+    assert rc == 0
+    st.n = nn.value
+    return stats.reshape(-1, 3)
