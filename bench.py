@@ -37,6 +37,74 @@ def bytes_per_particle_step(n, cells, nb, S, K):
     return 208 + 8 * cells / n + K * (32 + 12 * nb / n) + 24 * S
 
 
+def upsample_phase(args, lib, torch, C, np):
+    """Plain-upsampler throughput: 250k coarse particles (r = 0.02, 63^3
+    lattice in a 3 m box) with 8 fine particles each (2,000,376), fine
+    offsets U(+-0.02).  Each timed step = coarse gg_step + fine cell sort +
+    upsampler pass; fine-particles/s counts the fine work (sort + pass) from
+    the stage timers.  Upsampler algorithmic bytes per launch:
+    48 N_f (read+write fine pos, vel) + 24 N (coarse pos, vel) + 12 N_b +
+    8 C (two cell tables)."""
+    from granular_gpu_amd import _lib, scenes
+    from granular_gpu_amd.device import FineSet, PlainSim
+
+    side, box, per = args.upsample_side, 3.0, 8
+    scene = scenes.plain_box(side, box, dt=args.dt, g=-9.8, r=0.02)
+    n, nb = scene.n, scene.boundary.shape[0]
+    cells = scene.cells[0] * scene.cells[1] * scene.cells[2]
+    sim = PlainSim(scene, stab=3, solver=5)
+    fine = scenes.fine_cloud(scene.pos, per, 0.02, seed=7)
+    fs = FineSet(sim, fine)
+    nf = fs.n
+    print(f"# upsample scene: coarse={n} fine={nf} walls={nb} cells={cells}", file=sys.stderr,
+          flush=True)
+
+    def run(k):
+        for _ in range(k):
+            sim.step()
+            fs.sort()
+            fs.upsample()
+
+    run(args.upsample_warmup)
+    torch.cuda.synchronize()
+    lib.gg_profiler_enable(args.upsample_steps * 24 + 16)
+    t0 = time.perf_counter()
+    run(args.upsample_steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    tot = (C.c_double * len(_lib.STAGES))()
+    cnt = (C.c_int * len(_lib.STAGES))()
+    _lib.check("gg_profiler_collect", lib.gg_profiler_collect(tot, cnt, len(_lib.STAGES)))
+    lib.gg_profiler_enable(0)
+    st = {}
+    for k, name in enumerate(_lib.STAGES):
+        if cnt[k]:
+            st[name] = {"total_ms": round(tot[k], 4), "launches": cnt[k],
+                        "avg_us": round(1000.0 * tot[k] / cnt[k], 3)}
+    fp, fv = fs.host()
+    assert np.isfinite(fp).all() and np.isfinite(fv).all(), "non-finite fine state"
+    up_us, sort_us = st["upsample"]["avg_us"], st["upsample_sort"]["avg_us"]
+    per_launch = 48 * nf + 24 * n + 12 * nb + 8 * cells
+    ach = per_launch / (up_us * 1e-6) / 1e9
+    print(f"# upsample timed {args.upsample_steps} steps in {el:.3f} s", file=sys.stderr,
+          flush=True)
+    return {
+        "workload": f"{side}^3 coarse lattice r=0.02 (m=8) in a {box}^3 m box, {per} fine "
+                    f"per coarse (offsets U(+-0.02), seed 7), {scene.cells[0]}^3 cells, "
+                    f"{nb} walls, coarse 3+5 sweeps, steps {args.upsample_warmup}.."
+                    f"{args.upsample_warmup + args.upsample_steps}",
+        "coarse": n, "fine": nf, "steps": args.upsample_steps,
+        "fine_particles_per_s": round(nf / ((up_us + sort_us) * 1e-6), 1),
+        "fine_particles_per_s_pass_only": round(nf / (up_us * 1e-6), 1),
+        "coarse_plus_fine_ms_per_step": round(1000.0 * el / args.upsample_steps, 4),
+        "stages": st,
+        "roofline": {"kernel": "k_upsample (plain upsampler pass)", "bound": "hbm",
+                     "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": up_us},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +116,10 @@ def main():
     ap.add_argument("--cpu-baseline-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stage-timing", action="store_true")
+    ap.add_argument("--no-upsample", action="store_true")
+    ap.add_argument("--upsample-side", type=int, default=63, help="63 -> 250,047 coarse")
+    ap.add_argument("--upsample-steps", type=int, default=100)
+    ap.add_argument("--upsample-warmup", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 512 MiB device-to-device copy (calibrates HBM counters)")
     args = ap.parse_args()
@@ -146,6 +218,10 @@ def main():
     pos, vel, mass = sim.host()
     assert np.isfinite(pos).all() and np.isfinite(vel).all(), "non-finite state"
 
+    upsample = None
+    if not args.no_upsample:
+        upsample = upsample_phase(args, lib, torch, C, np)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import pyoracle as po
@@ -193,6 +269,7 @@ def main():
         "stages": stage_ms,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "upsample": upsample,
     }
     print(json.dumps(out), flush=True)
 

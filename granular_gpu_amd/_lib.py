@@ -20,7 +20,8 @@ GG_ERR_HIP = -3
 GG_ERR_WORKSPACE = -4
 GG_ERR_NO_DEVICE = -5
 GG_FLAG_NO_NLIST = 1
-STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep", "adaptive"]
+STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep", "adaptive", "upsample",
+          "upsample_sort", "mlp"]
 
 
 class gg_grid(C.Structure):
@@ -51,6 +52,13 @@ class gg_particles(C.Structure):
 class gg_adaptive(C.Structure):
     _fields_ = [("remove", C.c_void_p), ("merge", C.c_void_p), ("merge_velocity", C.c_void_p),
                 ("merge_count", C.c_void_p)]
+
+
+class gg_upsample_params(C.Structure):
+    _fields_ = [(k, C.c_float) for k in (
+        "dt", "gx", "gy", "gz", "r9", "boundary_radius", "repulsion", "min_weight",
+        "near_damping", "ground_y", "ground_damping", "max_velocity", "floor_y",
+        "floor_friction")]
 
 
 class gg_workspace(C.Structure):
@@ -92,6 +100,10 @@ _PROTOS = {
     "gg_adaptive_sampling": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, P(gg_particles), VP,
                                        P(gg_params), C.c_float, C.c_int, C.c_int,
                                        P(gg_workspace), VP, P(C.c_int)]),
+    "gg_upsample_params_default": (None, [P(gg_upsample_params)]),
+    "gg_upsampled_sort": (C.c_int, [P(gg_particles), VP, P(gg_workspace), VP]),
+    "gg_upsample_plain": (C.c_int, [P(gg_particles), P(gg_particles), VP, P(gg_particles), VP,
+                                    gg_grid, P(gg_upsample_params), C.c_int, VP]),
     "gg_profiler_enable": (C.c_int, [C.c_int]),
     "gg_profiler_collect": (C.c_int, [P(C.c_double), P(C.c_int), C.c_int]),
 }

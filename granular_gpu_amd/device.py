@@ -205,3 +205,47 @@ class AdaptiveSim(PlainSim):
         return dict(pos=g(self.pos), vel=g(self.vel), mass=g(self.mass), adl=g(self.adl),
                     remove=g(self.remove), merge=g(self.merge), mcount=g(self.mcount),
                     surface=g(self.surface), sd=g(self.sd))
+
+
+class FineSet:
+    """Fine (upsampled) particles of one scene: positions, velocities, their
+    own cell table and a sort-only workspace (the reference's _upsampled set
+    and _cell_start_upsampled, GranularSystem.hpp:60-90)."""
+
+    def __init__(self, sim: PlainSim, pos: np.ndarray, vel: np.ndarray | None = None):
+        lib = sim.lib
+        self.lib, self.sim = lib, sim
+        dev = sim.dev
+        n = int(pos.shape[0])
+        self.n = n
+        self.pos = torch.from_numpy(np.ascontiguousarray(pos, np.float32)).to(dev)
+        v = np.zeros_like(pos, np.float32) if vel is None else vel
+        self.vel = torch.from_numpy(np.ascontiguousarray(v, np.float32)).to(dev)
+        self.p2c = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.cs = torch.zeros(sim.C + 2, dtype=torch.int32, device=dev)
+        nbytes = lib.gg_workspace_bytes(max(n, 1), 0, sim.grid, 0, 0, 0)
+        if nbytes == 0:
+            raise ValueError("invalid fine workspace request")
+        self.ws_buf = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        self.ws = _lib.gg_workspace(self.ws_buf.data_ptr(), int(nbytes), max(n, 1), 0, sim.grid,
+                                    0, 0, 0)
+        check("gg_workspace_init", lib.gg_workspace_init(C.byref(self.ws), _stream()))
+        self.set = _lib.gg_particles(n, max(n, 1), _ptr(self.pos), _ptr(self.vel), None, None,
+                                     None, None, None, _ptr(self.p2c), None, None)
+        self.up = _lib.gg_upsample_params()
+        lib.gg_upsample_params_default(C.byref(self.up))
+
+    def sort(self):
+        check("gg_upsampled_sort", self.lib.gg_upsampled_sort(C.byref(self.set), _ptr(self.cs),
+                                                              C.byref(self.ws), _stream()))
+
+    def upsample(self, flags=0):
+        s = self.sim
+        check("gg_upsample_plain",
+              self.lib.gg_upsample_plain(C.byref(self.set), C.byref(s.set), _ptr(s.cs),
+                                         C.byref(s.bset), _ptr(s.bcs), s.grid, C.byref(self.up),
+                                         flags, _stream()))
+
+    def host(self):
+        torch.cuda.synchronize()
+        return self.pos.cpu().numpy().copy(), self.vel.cpu().numpy().copy()

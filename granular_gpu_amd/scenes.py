@@ -120,12 +120,19 @@ class Scene:
         return self.pos.shape[0]
 
 
+def radius_mass(r: float) -> np.float32:
+    """Mass of a particle of radius r at the scenes' density (m = 1 at
+    r = 0.01): m = (r / 0.01)^3, formed in float64 and rounded once."""
+    return np.float32((float(r) / 0.01) ** 3)
+
+
 def plain_box(n_side: int = 32, box: float = 1.0, seed: int = 1, r: float = 0.01,
               dt: float = 1.0 / 240.0, g: float = -9.81) -> Scene:
     """Plain box scene: n_side^3 particles of radius r on a 2r lattice
     (jitter U(+-0.1r), seeded), centred in x/z, bottoms 0.1 above the floor
     surface (floor particles at 0.005*box with radius 0.01), BOX walls.
 
+    Masses follow the radius (radius_mass).
     n_side=32, box=1   -> 32,768 particles, 25^3 cells, 14,408 walls.
     n_side=100, box=2.4 -> 1,000,000 particles, 60^3 cells, 84,968 walls."""
     space = (box, box, box)
@@ -139,6 +146,19 @@ def plain_box(n_side: int = 32, box: float = 1.0, seed: int = 1, r: float = 0.01
     walls = box_walls(space, CELL_LENGTH, "box")
     return Scene(
         name=f"plain_box_{n_side}^3_in_{box}m", pos=pos, vel=np.zeros((n, 3), np.float32),
-        mass=np.ones(n, np.float32), boundary=walls, space=space, cell_length=CELL_LENGTH,
+        mass=np.full(n, radius_mass(r), np.float32), boundary=walls, space=space,
+        cell_length=CELL_LENGTH,
         cells=cell_counts(space), dt=np.float32(dt), gravity=(0.0, float(np.float32(g)), 0.0),
         density=DENSITY_R001, radius=r)
+
+
+def fine_cloud(coarse: np.ndarray, per: int, radius: float, seed: int = 7) -> np.ndarray:
+    """`per` fine particles around each coarse centre, offsets uniform in
+    [-radius, radius) per axis (the reference draws 25 per coarse particle
+    the same way, main.cu:804-816, with its own generator).  Fine particles
+    of one coarse particle are contiguous, in coarse order."""
+    c = np.asarray(coarse, np.float32)
+    n = c.shape[0]
+    u = uniform01(seed, n * per * 3).reshape(n, per, 3)
+    off = ((2.0 * u - 1.0) * float(radius)).astype(np.float32)
+    return (c[:, None, :] + off).reshape(n * per, 3).astype(np.float32)

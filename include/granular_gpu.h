@@ -196,6 +196,39 @@ int gg_adaptive_sampling(gg_particles *set, gg_adaptive *ad, const int *cell_sta
                          int blend_steps, const gg_workspace *ws,
                          hipStream_t stream, int *stats);
 
+/* ---- upsampling ----------------------------------------------------------- */
+/* Constants of update_upsampled_cuda (hard-coded in the reference:
+ * Solver.cu:29, 1259-1264, 1313-1359); gg_upsample_params_default fills them. */
+typedef struct gg_upsample_params {
+  float dt, gx, gy, gz;   /* 0.002, (0,-9.8,0) */
+  float r9;               /* 1/h^2 = 493.8271605 (h = 0.045) */
+  float boundary_radius;  /* 0.015 */
+  float repulsion;        /* 3.0 */
+  float min_weight;       /* 1e-6 */
+  float near_damping;     /* 0.8 */
+  float ground_y;         /* 0.1 */
+  float ground_damping;   /* 0.5 */
+  float max_velocity;     /* 7.0 */
+  float floor_y;          /* 0.04 */
+  float floor_friction;   /* 0.95 */
+} gg_upsample_params;
+void gg_upsample_params_default(gg_upsample_params *u);
+
+/* Fine-particle cell sort (neighbor_search_upsampled,
+ * GranularSystem.cu:282-395): pos and vel permuted by one stable sort;
+ * particle_2_cell keeps the pre-sort keys; fine_cell_start (C+2) built.
+ * fine_ws is a workspace whose capacity covers the fine set. */
+int gg_upsampled_sort(gg_particles *fine, int *fine_cell_start,
+                      const gg_workspace *fine_ws, hipStream_t stream);
+
+/* One pass of update_upsampled_cuda (Solver.cu:1244-1387): fine pos/vel
+ * updated in place from the coarse pos/vel, their cell table (start of the
+ * step, as in the reference) and the boundary set.  No workspace needed. */
+int gg_upsample_plain(gg_particles *fine, const gg_particles *coarse,
+                      const int *cell_start, const gg_particles *boundary,
+                      const int *bcell_start, gg_grid grid,
+                      const gg_upsample_params *up, int flags, hipStream_t stream);
+
 /* Stage entry points of gg_step, in order (for stage-level checking).
  * gg_predict: T7-T9 (Solver.cu:125-191) on the sorted set; builds the
  *   neighbour lists unless GG_FLAG_NO_NLIST.
@@ -228,7 +261,10 @@ enum {
   GG_STAGE_FINALIZE = 4, /* final update */
   GG_STAGE_STAB = 5,     /* one stabilisation contact sweep */
   GG_STAGE_ADAPTIVE = 6, /* surface / distance / merge / split passes */
-  GG_STAGE_COUNT = 7
+  GG_STAGE_UPSAMPLE = 7, /* plain upsampler pass */
+  GG_STAGE_UPSAMPLE_SORT = 8, /* fine-particle cell sort */
+  GG_STAGE_MLP = 9,      /* learned upsampler (features + MLP) */
+  GG_STAGE_COUNT = 10
 };
 int gg_profiler_enable(int max_records); /* 0 disables */
 int gg_profiler_collect(double *total_ms, int *count, int nstages);

@@ -135,6 +135,28 @@ int oracle_adaptive_steps(int *n, int cap, float *pos, float *vel, float *mass,
                           int nsteps, int merge_first, float max_mass,
                           int *stats);
 
+/* ---- plain upsampler (update_upsampled_cuda, Solver.cu:1244-1369) ------ */
+typedef struct oracle_upsample_params {
+  float dt, gx, gy, gz;    /* 0.002, (0,-9.8,0): hard-coded in the reference */
+  float r9;                /* 1/h^2 = 493.8271605                             */
+  float boundary_radius;   /* 0.015 */
+  float repulsion;         /* 3.0   */
+  float min_weight;        /* 1e-6  */
+  float near_damping;      /* 0.8   */
+  float ground_y;          /* 0.1   */
+  float ground_damping;    /* 0.5   */
+  float max_velocity;      /* 7.0   */
+  float floor_y;           /* 0.04  */
+  float floor_friction;    /* 0.95  */
+} oracle_upsample_params;
+void oracle_upsample_params_default(oracle_upsample_params *u);
+/* One upsampler pass over nf fine particles (in place) against the coarse
+ * positions/velocities and their cell table (stencil base floor(x/L)).    */
+int oracle_upsample_plain(int nf, float *fpos, float *fvel, int n,
+                          const float *cpos, const float *cvel, const int *cs,
+                          int nb, const float *bpos, const int *bcs,
+                          const oracle_params *p, const oracle_upsample_params *u);
+
 /* Radius helpers (three reference variants), exposed for unit tests.     */
 float oracle_radius_contact(float m, float density);
 float oracle_radius_surface(float m, float density);

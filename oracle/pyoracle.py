@@ -27,6 +27,13 @@ class oracle_params(C.Structure):
     ]
 
 
+class oracle_upsample_params(C.Structure):
+    _fields_ = [(k, C.c_float) for k in (
+        "dt", "gx", "gy", "gz", "r9", "boundary_radius", "repulsion", "min_weight",
+        "near_damping", "ground_y", "ground_damping", "max_velocity", "floor_y",
+        "floor_friction")]
+
+
 _lib = None
 
 
@@ -58,6 +65,10 @@ def load() -> C.CDLL:
     lib.oracle_nlist_f32.argtypes = [C.c_int, fp, fp, C.c_int, fp, ip, ip, P, C.c_int, C.c_int,
                                      ip, ip, ip, ip, ip]
     vp = C.c_void_p
+    lib.oracle_upsample_params_default.argtypes = [C.POINTER(oracle_upsample_params)]
+    lib.oracle_upsample_params_default.restype = None
+    lib.oracle_upsample_plain.argtypes = [C.c_int, fp, fp, C.c_int, fp, fp, ip, C.c_int, fp, ip, P,
+                                          C.POINTER(oracle_upsample_params)]
     lib.oracle_surface.argtypes = [C.c_int, fp, fp, ip, P, fp, ip]
     lib.oracle_surface_distance.argtypes = [C.c_int, fp, ip, ip, P, fp]
     lib.oracle_merge_mark.argtypes = [C.c_int, fp, fp, fp, ip, fp, ip, fp, fp, ip, P, C.c_float,
@@ -213,3 +224,26 @@ def adaptive_steps(st: AdaptiveState, bpos_sorted, bcs, params, nsteps, merge_fi
     assert rc == 0
     st.n = nn.value
     return stats.reshape(-1, 3)
+
+
+def upsample_params():
+    u = oracle_upsample_params()
+    load().oracle_upsample_params_default(C.byref(u))
+    return u
+
+
+def upsample_plain(fpos, fvel, cpos, cvel, cs, bpos_sorted, bcs, params, u=None, steps=1):
+    """In-place copies: returns updated (fpos, fvel) after `steps` passes
+    against fixed coarse data."""
+    lib = load()
+    u = u or upsample_params()
+    fp_ = np.ascontiguousarray(fpos, np.float32).copy()
+    fv_ = np.ascontiguousarray(fvel, np.float32).copy()
+    for _ in range(steps):
+        lib.oracle_upsample_plain(fp_.shape[0], fp_.reshape(-1), fv_.reshape(-1), cpos.shape[0],
+                                  np.ascontiguousarray(cpos, np.float32).reshape(-1),
+                                  np.ascontiguousarray(cvel, np.float32).reshape(-1), cs,
+                                  bpos_sorted.shape[0],
+                                  np.ascontiguousarray(bpos_sorted, np.float32).reshape(-1), bcs,
+                                  C.byref(params), C.byref(u))
+    return fp_, fv_
