@@ -33,8 +33,22 @@ enum MergeState : int {
   M_PARTNER_LOCK = 6  // partner locked (-1) but not merged, self released
 };
 
+// Approximate 1/sqrt(x): Lomont's bit-level estimate ("Fast Inverse Square
+// Root", 2003) and two Newton-Raphson steps, relative error below 5e-6.
+// The reference normalizes with the device's approximate rsqrtf; an
+// approximation (rather than a correctly rounded 1/sqrt) keeps the cover
+// vector of a regular-lattice interior particle nonzero, as there.
+__device__ __forceinline__ float rsqrt_approx(float x) {
+  float y = __uint_as_float(0x5f3759dfu - (__float_as_uint(x) >> 1));
+  const float hx = 0.5f * x;
+  y = y * (1.5f - hx * y * y);
+  y = y * (1.5f - hx * y * y);
+  return y;
+}
+
+// normalize(v) = v * rsqrt(v.v) (helper_math.h:1325-1329); 0 for v = 0.
 __device__ __forceinline__ float3 nrm(float x, float y, float z) {
-  const float inv = 1.0f / sqrtf(x * x + y * y + z * z);
+  const float inv = rsqrt_approx(x * x + y * y + z * z);
   return make_float3(x * inv, y * inv, z * inv);
 }
 

@@ -1,0 +1,223 @@
+"""ctypes bindings of the host C++ API's C-ABI (libgranular_host.so, declared
+in include/granular_host.h): the GranularSystem class behind a handle, the
+scene JSON reader and the scene builders.
+
+HostSystem mirrors the reference's GranularSystem (GranularSystem.hpp:8-56):
+the constructor runs one step, step() returns 1 and frame times accumulate.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "libgranular_host.so"
+_lib = None
+
+FIELDS = {"pos": (0, np.float32, 3), "vel": (1, np.float32, 3), "mass": (2, np.float32, 1),
+          "particle_2_cell": (3, np.int32, 1), "surface": (4, np.int32, 1),
+          "surface_distance": (5, np.float32, 1), "adaptive_last_step": (6, np.int32, 1),
+          "is_animated": (7, np.int32, 1)}
+SETS = {"granular": 0, "boundary": 1, "fine": 2}
+
+
+class ggh_scene(C.Structure):
+    _fields_ = [("space_size", C.c_float * 3), ("dt", C.c_float), ("gravity", C.c_float * 3),
+                ("sph_spacing", C.c_float), ("init_spacing", C.c_float),
+                ("smoothing_radius", C.c_float), ("cell_length", C.c_float),
+                ("cell_size", C.c_int * 3), ("density", C.c_float), ("box_size", C.c_float * 3),
+                ("boundary_translation", C.c_float * 3), ("particle_translation", C.c_float * 3),
+                ("scene", C.c_int), ("is_adaptive", C.c_int), ("obj_file", C.c_char * 256),
+                ("save_file", C.c_char * 256)]
+
+
+class HostError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Loads libgranular_host.so (built by __graft_entry__.build()); raises
+    when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise HostError(f"{_LIB_PATH} is not built; run __graft_entry__.build()")
+    from . import _lib as gpu
+    gpu.load()  # resolves libgranular_gpu.so first
+    lib = C.CDLL(str(_LIB_PATH))
+    fp, ip, vp = C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_void_p
+    S = C.POINTER(ggh_scene)
+    sig = {
+        "ggh_last_error": (C.c_char_p, []),
+        "ggh_load_scene": (C.c_int, [C.c_char_p, S]),
+        "ggh_parse_scene": (C.c_int, [C.c_char_p, S]),
+        "ggh_scene_lattice": (C.c_int, [S, vp, C.c_int]),
+        "ggh_scene_walls": (C.c_int, [S, vp, C.c_int]),
+        "ggh_upsampled_cloud": (C.c_int, [vp, C.c_int, C.c_float, C.c_int, vp, C.c_int]),
+        "ggh_system_create": (vp, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, fp, C.c_float,
+                                   C.c_float, fp, ip, C.c_float, C.c_float, C.c_int, C.c_int,
+                                   C.c_int]),
+        "ggh_system_destroy": (None, [vp]),
+        "ggh_system_step": (C.c_int, [vp, C.c_int]),
+        "ggh_system_size": (C.c_int, [vp, C.c_int]),
+        "ggh_system_read": (C.c_int, [vp, C.c_int, C.c_int, vp, C.c_int]),
+        "ggh_system_translate_animated": (C.c_int, [vp, C.c_float, C.c_float, C.c_float]),
+        "ggh_system_frame_times": (C.c_int, [vp, fp, C.c_int]),
+        "ggh_system_particle_stats": (C.c_int, [vp, ip, C.c_int]),
+        "ggh_system_save_stats": (C.c_int, [vp, C.c_char_p]),
+        "ggh_system_set_solver": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float,
+                                            C.c_int]),
+        "ggh_darray_selftest": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _err(lib) -> str:
+    m = lib.ggh_last_error()
+    return m.decode() if m else "unknown error"
+
+
+def _chk(lib, rc, what):
+    if rc is None or (isinstance(rc, int) and rc < 0):
+        raise HostError(f"{what}: {_err(lib)}")
+    return rc
+
+
+def _f3(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, 3))
+
+
+def load_scene(path) -> ggh_scene:
+    lib = load()
+    s = ggh_scene()
+    _chk(lib, lib.ggh_load_scene(str(path).encode(), C.byref(s)), "ggh_load_scene")
+    return s
+
+
+def parse_scene(text: str) -> ggh_scene:
+    lib = load()
+    s = ggh_scene()
+    _chk(lib, lib.ggh_parse_scene(text.encode(), C.byref(s)), "ggh_parse_scene")
+    return s
+
+
+def _emit(fn, *args) -> np.ndarray:
+    lib = load()
+    n = _chk(lib, fn(*args, None, 0), fn.__name__)
+    out = np.zeros((n, 3), np.float32)
+    if n:
+        _chk(lib, fn(*args, out.ctypes.data, n), fn.__name__)
+    return out
+
+
+def scene_lattice(s: ggh_scene) -> np.ndarray:
+    return _emit(load().ggh_scene_lattice, C.byref(s))
+
+
+def scene_walls(s: ggh_scene) -> np.ndarray:
+    return _emit(load().ggh_scene_walls, C.byref(s))
+
+
+def upsampled_cloud(coarse, radius: float, per: int = 25) -> np.ndarray:
+    c = _f3(coarse)
+    return _emit(load().ggh_upsampled_cloud, c.ctypes.data, c.shape[0], float(radius), int(per))
+
+
+def darray_selftest(n: int) -> str:
+    lib = load()
+    buf = C.create_string_buffer(512)
+    rc = lib.ggh_darray_selftest(int(n), buf, 512)
+    msg = buf.value.decode()
+    if rc != 0:
+        raise HostError(f"DArray self-test (n={n}) failed: {msg}")
+    return msg
+
+
+class HostSystem:
+    """One GranularSystem (the reference's host class) behind its C handle."""
+
+    def __init__(self, pos, boundary, fine, space_size, cell_length, dt, gravity, cell_size,
+                 density, upsampled_radius=0.005, is_move_boundary=False, is_adaptive=0,
+                 boundary_animated=None, verbose=False):
+        lib = load()
+        self.lib = lib
+        p, b, f = _f3(pos), _f3(boundary), _f3(fine)
+        anim = None
+        if boundary_animated is not None:
+            anim = np.ascontiguousarray(boundary_animated, np.int32)
+        sp = (C.c_float * 3)(*[float(v) for v in space_size])
+        g = (C.c_float * 3)(*[float(v) for v in gravity])
+        cs = (C.c_int * 3)(*[int(v) for v in cell_size])
+        h = lib.ggh_system_create(p.ctypes.data, p.shape[0], b.ctypes.data, b.shape[0],
+                                  None if anim is None else anim.ctypes.data, f.ctypes.data,
+                                  f.shape[0], sp, float(cell_length), float(dt), g, cs,
+                                  float(density), float(upsampled_radius),
+                                  int(bool(is_move_boundary)), int(is_adaptive), int(verbose))
+        if not h:
+            raise HostError(f"ggh_system_create: {_err(lib)}")
+        self.h = C.c_void_p(h)
+
+    @classmethod
+    def from_scene(cls, scene, fine=None, **kw):
+        """From a granular_gpu_amd.scenes.Scene (fine particles optional)."""
+        f = np.zeros((0, 3), np.float32) if fine is None else fine
+        return cls(scene.pos, scene.boundary, f, scene.space, scene.cell_length, scene.dt,
+                   scene.gravity, scene.cells, scene.density, **kw)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ggh_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1) -> int:
+        return _chk(self.lib, self.lib.ggh_system_step(self.h, int(nsteps)), "ggh_system_step")
+
+    def size(self, which="granular") -> int:
+        k = {"granular": 0, "boundary": 1, "fine": 2, "total": 3}[which]
+        return self.lib.ggh_system_size(self.h, k)
+
+    def read(self, which: str, field: str, count: int | None = None) -> np.ndarray:
+        code, dt, w = FIELDS[field]
+        n = self.size(which) if count is None else count
+        out = np.zeros((n, w) if w > 1 else n, dt)
+        _chk(self.lib, self.lib.ggh_system_read(self.h, SETS[which], code, out.ctypes.data, n),
+             "ggh_system_read")
+        return out
+
+    def translate_animated(self, d) -> int:
+        return _chk(self.lib, self.lib.ggh_system_translate_animated(self.h, *map(float, d)),
+                    "ggh_system_translate_animated")
+
+    def frame_times(self) -> np.ndarray:
+        n = self.lib.ggh_system_frame_times(self.h, None, 0)
+        out = np.zeros(n, np.float32)
+        self.lib.ggh_system_frame_times(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), n)
+        return out
+
+    def particle_stats(self) -> np.ndarray:
+        n = self.lib.ggh_system_particle_stats(self.h, None, 0)
+        out = np.zeros((n, 3), np.int32)
+        self.lib.ggh_system_particle_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int)), n)
+        return out
+
+    def save_stats(self, path):
+        _chk(self.lib, self.lib.ggh_system_save_stats(self.h, str(path).encode()),
+             "ggh_system_save_stats")
+
+    def set_solver(self, stab=3, solver=5, mu_s=0.0, mu_k=0.0, flags=0):
+        _chk(self.lib, self.lib.ggh_system_set_solver(self.h, stab, solver, mu_s, mu_k, flags),
+             "ggh_system_set_solver")

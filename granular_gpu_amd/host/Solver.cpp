@@ -1,0 +1,317 @@
+// Solver of the host API over the library's C-ABI.
+#include "granular/Solver.hpp"
+
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+
+namespace {
+
+void check(int rc, const char *what) {
+  if (rc != GG_OK)
+    throw std::runtime_error(std::string(what) + ": " + gg_status_string(rc));
+}
+
+constexpr int kGmax = 48;  // granular candidates per neighbour list
+constexpr int kBmax = 16;  // boundary candidates per neighbour list
+
+}  // namespace
+
+Solver::Solver(const std::shared_ptr<GranularParticles> &particles)
+    : _max_iter(5),
+      _blend_factor(40),
+      _buffer_remove(particles->size()),
+      _buffer_merge(particles->size()),
+      _buffer_merge_velocity(particles->size()),
+      _buffer_merge_count(particles->size()) {
+  _ws = gg_workspace{};
+}
+
+void Solver::set_iterations(int stab, int solver) {
+  if (stab < 0 || solver < 0) throw std::invalid_argument("iteration counts must be >= 0");
+  _stab = stab;
+  _solver_iters = solver;
+}
+
+void Solver::set_friction(float mu_s, float mu_k) {
+  _mu_s = mu_s;
+  _mu_k = mu_k;
+}
+
+void Solver::set_flags(int flags) { _flags = flags; }
+
+void Solver::reserve(unsigned int cap) {
+  _buffer_remove.reserve(cap);
+  _buffer_merge.reserve(cap);
+  _buffer_merge_velocity.reserve(cap);
+  _buffer_merge_count.reserve(cap);
+}
+
+gg_params Solver::params(float dt, float3 G, float density) const {
+  gg_params p;
+  gg_params_default(&p);
+  p.dt = dt;
+  p.gx = G.x;
+  p.gy = G.y;
+  p.gz = G.z;
+  p.density = density;
+  p.stab_iters = _stab;
+  p.solver_iters = _solver_iters;
+  p.mu_s = _mu_s;
+  p.mu_k = _mu_k;
+  p.flags = _flags;
+  return p;
+}
+
+gg_adaptive Solver::adaptive_view() {
+  gg_adaptive a;
+  a.remove = _buffer_remove.addr();
+  a.merge = _buffer_merge.addr();
+  a.merge_velocity = reinterpret_cast<gg_float3 *>(_buffer_merge_velocity.addr());
+  a.merge_count = _buffer_merge_count.addr();
+  return a;
+}
+
+gg_grid Solver::grid(int3 cell_size, float cell_length) const {
+  gg_grid g;
+  g.cell_length = cell_length;
+  g.cx = cell_size.x;
+  g.cy = cell_size.y;
+  g.cz = cell_size.z;
+  return g;
+}
+
+void Solver::ensure_workspace(int cap, int bcap, gg_grid g) {
+  const int sweeps = _stab + _solver_iters;
+  const bool same_grid = _ws_ready && _ws.grid.cx == g.cx && _ws.grid.cy == g.cy &&
+                         _ws.grid.cz == g.cz && _ws.grid.cell_length == g.cell_length;
+  if (same_grid && _ws.capacity >= cap && _ws.boundary_capacity >= bcap &&
+      _ws.max_sweeps >= sweeps)
+    return;
+  gg_workspace w{};
+  w.capacity = same_grid ? std::max(cap, _ws.capacity) : cap;
+  w.boundary_capacity = same_grid ? std::max(bcap, _ws.boundary_capacity) : bcap;
+  w.grid = g;
+  w.max_sweeps = same_grid ? std::max(sweeps, _ws.max_sweeps) : sweeps;
+  w.gmax = kGmax;
+  w.bmax = kBmax;
+  const size_t bytes = gg_workspace_bytes(w.capacity, w.boundary_capacity, g, w.max_sweeps,
+                                          w.gmax, w.bmax);
+  if (bytes == 0) throw std::runtime_error("invalid workspace request");
+  _ws_mem.reset();
+  _ws_mem.reset(new DArray<unsigned int>((unsigned int)((bytes + 3) / 4)));
+  w.base = _ws_mem->addr();
+  w.bytes = bytes;
+  check(gg_workspace_init(&w, nullptr), "gg_workspace_init");
+  _ws = w;
+  _ws_ready = true;
+  // the packed boundary lives in the workspace: rebuild it
+  if (_bnd && _bcs) {
+    gg_particles b = _bnd->view();
+    check(gg_boundary_build(&b, _bcs->addr(), &_ws, nullptr), "gg_boundary_build");
+  }
+}
+
+const gg_workspace &Solver::workspace(const std::shared_ptr<GranularParticles> &particles,
+                                      const std::shared_ptr<GranularParticles> &boundary,
+                                      const DArray<int> *cell_start_boundary, int3 cell_size,
+                                      float cell_length) {
+  if (boundary && cell_start_boundary) {
+    _bnd = boundary;
+    _bcs = cell_start_boundary;
+  }
+  int cap = (int)std::min({_buffer_remove.capacity(), _buffer_merge.capacity(),
+                           _buffer_merge_velocity.capacity(), _buffer_merge_count.capacity()});
+  if (particles) cap = std::min(cap, (int)particles->device_capacity());
+  const int bcap = _bnd ? (int)std::max(_bnd->size(), 1u) : 1;
+  ensure_workspace(std::max(cap, 1), bcap, grid(cell_size, cell_length));
+  return _ws;
+}
+
+void Solver::sort_boundary(const std::shared_ptr<GranularParticles> &boundary,
+                           DArray<int> &cell_start_boundary, int3 cell_size, float cell_length) {
+  _bnd = boundary;
+  _bcs = &cell_start_boundary;
+  const int cap = (int)std::max(std::min({_buffer_remove.capacity(), _buffer_merge.capacity(),
+                                          _buffer_merge_velocity.capacity(),
+                                          _buffer_merge_count.capacity()}),
+                                1u);
+  ensure_workspace(cap, (int)std::max(boundary->size(), 1u), grid(cell_size, cell_length));
+  gg_particles b = boundary->view();
+  check(gg_boundary_build(&b, cell_start_boundary.addr(), &_ws, nullptr), "gg_boundary_build");
+}
+
+void Solver::sort_particles(std::shared_ptr<GranularParticles> &particles,
+                            const std::shared_ptr<GranularParticles> &boundary,
+                            DArray<int> &cell_start_particle, int3 cell_size, float cell_length) {
+  const gg_workspace &ws = workspace(particles, boundary, nullptr, cell_size, cell_length);
+  gg_particles s = particles->view();
+  gg_adaptive a = adaptive_view();
+  check(gg_grid_build(&s, &a, cell_start_particle.addr(), nullptr, &ws, nullptr), "gg_grid_build");
+}
+
+void Solver::predict(std::shared_ptr<GranularParticles> &particles,
+                     const std::shared_ptr<GranularParticles> &boundary,
+                     const DArray<int> &cell_start_particle,
+                     const DArray<int> &cell_start_boundary, int3 cell_size, float cell_length,
+                     float dt, float3 G, const float density) {
+  const gg_workspace &ws =
+      workspace(particles, boundary, &cell_start_boundary, cell_size, cell_length);
+  gg_particles s = particles->view();
+  gg_particles b = boundary->view();
+  const gg_params p = params(dt, G, density);
+  check(gg_predict(&s, cell_start_particle.addr(), &b, cell_start_boundary.addr(), &p, &ws,
+                   nullptr),
+        "gg_predict");
+  _last_dt = dt;
+  _last_G = G;
+  _last_density = density;
+}
+
+void Solver::project(std::shared_ptr<GranularParticles> &particles,
+                     const std::shared_ptr<GranularParticles> &boundaries,
+                     const DArray<int> &cell_start_granular,
+                     const DArray<int> &cell_start_boundary, int3 cell_size, float3 space_size,
+                     float cell_length, int max_iter, const float density) {
+  (void)space_size;
+  const int saved = _solver_iters;
+  _solver_iters = max_iter;
+  try {
+    const gg_workspace &ws =
+        workspace(particles, boundaries, &cell_start_boundary, cell_size, cell_length);
+    gg_particles s = particles->view();
+    gg_particles b = boundaries->view();
+    const gg_params p = params(_last_dt, _last_G, density);
+    for (int k = 0; k < _stab + max_iter; ++k)
+      check(gg_contact_sweep(&s, cell_start_granular.addr(), &b, cell_start_boundary.addr(), &p,
+                             k, &ws, nullptr),
+            "gg_contact_sweep");
+  } catch (...) {
+    _solver_iters = saved;
+    throw;
+  }
+  _solver_iters = saved;
+}
+
+void Solver::final_update(std::shared_ptr<GranularParticles> &particles, float dt) {
+  if (!_ws_ready) throw std::runtime_error("final_update before predict");
+  gg_particles s = particles->view();
+  const gg_params p = params(dt, _last_G, _last_density);
+  check(gg_finalize(&s, &p, &_ws, nullptr), "gg_finalize");
+}
+
+void Solver::step(std::shared_ptr<GranularParticles> &particles,
+                  const std::shared_ptr<GranularParticles> &boundary,
+                  const DArray<int> &cell_start_particle, const DArray<int> &cell_start_boundary,
+                  float3 space_size, int3 cell_size, float cell_length, float dt, float3 G,
+                  const float density) {
+  predict(particles, boundary, cell_start_particle, cell_start_boundary, cell_size, cell_length,
+          dt, G, density);
+  project(particles, boundary, cell_start_particle, cell_start_boundary, cell_size, space_size,
+          cell_length, _solver_iters, density);
+  final_update(particles, dt);
+}
+
+void Solver::sort_and_step(std::shared_ptr<GranularParticles> &particles,
+                           const std::shared_ptr<GranularParticles> &boundary,
+                           DArray<int> &cell_start_particle,
+                           const DArray<int> &cell_start_boundary, float3 space_size,
+                           int3 cell_size, float cell_length, float dt, float3 G,
+                           const float density) {
+  (void)space_size;
+  const gg_workspace &ws =
+      workspace(particles, boundary, &cell_start_boundary, cell_size, cell_length);
+  gg_particles s = particles->view();
+  gg_particles b = boundary->view();
+  gg_adaptive a = adaptive_view();
+  const gg_params p = params(dt, G, density);
+  check(gg_step(&s, &a, cell_start_particle.addr(), &b, cell_start_boundary.addr(), &p, &ws,
+                nullptr),
+        "gg_step");
+  _last_dt = dt;
+  _last_G = G;
+  _last_density = density;
+}
+
+void Solver::surface(std::shared_ptr<GranularParticles> &particles,
+                     const DArray<int> &cell_start_granular, int3 cell_size, float cell_length,
+                     const float density) {
+  const gg_workspace &ws = workspace(particles, nullptr, nullptr, cell_size, cell_length);
+  gg_particles s = particles->view();
+  const gg_params p = params(_last_dt, _last_G, density);
+  check(gg_surface(&s, cell_start_granular.addr(), &p, &ws, nullptr), "gg_surface");
+}
+
+void Solver::surface_distance(std::shared_ptr<GranularParticles> &particles,
+                              const DArray<int> &cell_start_granular, int3 cell_size,
+                              float cell_length) {
+  const gg_workspace &ws = workspace(particles, nullptr, nullptr, cell_size, cell_length);
+  gg_particles s = particles->view();
+  check(gg_surface_distance(&s, cell_start_granular.addr(), &ws, nullptr), "gg_surface_distance");
+}
+
+void Solver::adaptive_sampling(std::shared_ptr<GranularParticles> &particles,
+                               const std::shared_ptr<GranularParticles> &boundaries,
+                               const DArray<int> &cell_start_granular,
+                               const DArray<int> &cell_start_boundary, const float max_mass,
+                               int3 cell_size, float3 space_size, float cell_length,
+                               const float density) {
+  (void)space_size;
+  const int num = (int)particles->size();
+  if (num == 0) return;
+  const gg_workspace &ws =
+      workspace(particles, boundaries, &cell_start_boundary, cell_size, cell_length);
+  // merge is skipped on the first call (Solver.cu:910-922)
+  const int do_merge = (_t_merge_iter == 1) ? 1 : 0;
+  if (do_merge) _t_merge_iter = 0;
+  _t_merge_iter++;
+  gg_particles s = particles->view();
+  gg_particles b = boundaries->view();
+  gg_adaptive a = adaptive_view();
+  const gg_params p = params(_last_dt, _last_G, density);
+  int stats[3] = {0, 0, 0};
+  check(gg_adaptive_sampling(&s, &a, cell_start_granular.addr(), &b, cell_start_boundary.addr(),
+                             &p, max_mass, do_merge, _blend_factor, &ws, nullptr, stats),
+        "gg_adaptive_sampling");
+  particles->set_size((unsigned int)s.n);
+  _buffer_remove.set_length((unsigned int)s.n);
+  _buffer_merge.set_length((unsigned int)s.n);
+  _buffer_merge_count.set_length((unsigned int)s.n);
+  _particle_stats.push_back(std::make_tuple(stats[0], stats[1], num));
+}
+
+void Solver::upsampled_update(std::shared_ptr<GranularParticles> &particles,
+                              const std::shared_ptr<GranularParticles> &boundaries,
+                              std::shared_ptr<GranularParticles> &upsampled,
+                              const DArray<int> &cell_start_upsampled,
+                              const DArray<int> &cell_start_granular,
+                              const DArray<int> &cell_start_boundary, int3 cell_size,
+                              float3 space_size, float cell_length, const float density) {
+  (void)cell_start_upsampled;
+  (void)space_size;
+  (void)density;
+  gg_particles f = upsampled->view();
+  gg_particles c = particles->view();
+  gg_particles b = boundaries->view();
+  gg_upsample_params u;
+  gg_upsample_params_default(&u);
+  check(gg_upsample_plain(&f, &c, cell_start_granular.addr(), &b, cell_start_boundary.addr(),
+                          grid(cell_size, cell_length), &u, _flags, nullptr),
+        "gg_upsample_plain");
+}
+
+void Solver::save_particle_stats(const std::string &filename) {
+  std::ofstream outfile(filename);
+  if (!outfile.is_open()) {
+    std::cerr << "Failed to open file for particle statistics: " << filename << std::endl;
+    return;
+  }
+  outfile << "Frame,MergedParticles,SplitParticles,TotalParticles" << std::endl;
+  for (size_t i = 0; i < _particle_stats.size(); i++) {
+    const auto &st = _particle_stats[i];
+    outfile << i << "," << std::get<0>(st) << "," << std::get<1>(st) << "," << std::get<2>(st)
+            << std::endl;
+  }
+  outfile.close();
+  std::cout << "Particle statistics saved to " << filename << std::endl;
+}

@@ -1,0 +1,324 @@
+// C-ABI over the host C++ API (handles for ctypes and other FFI callers).
+#include "granular_host.h"
+
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "granular/GranularSystem.hpp"
+#include "granular/SceneConfig.hpp"
+
+struct ggh_system {
+  std::unique_ptr<GranularSystem> sys;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::exception &e) {
+  g_err = e.what();
+  return -1;
+}
+
+void copy_str(char *dst, size_t cap, const std::string &s) {
+  std::strncpy(dst, s.c_str(), cap - 1);
+  dst[cap - 1] = '\0';
+}
+
+void to_c(const SceneConfig &c, ggh_scene *o) {
+  std::memset(o, 0, sizeof(*o));
+  o->space_size[0] = c.space_size.x;
+  o->space_size[1] = c.space_size.y;
+  o->space_size[2] = c.space_size.z;
+  o->dt = c.dt;
+  o->gravity[0] = c.G.x;
+  o->gravity[1] = c.G.y;
+  o->gravity[2] = c.G.z;
+  o->sph_spacing = c.sphSpacing;
+  o->init_spacing = c.initSpacing;
+  o->smoothing_radius = c.smoothing_radius;
+  o->cell_length = c.cell_length;
+  o->cell_size[0] = c.cell_size.x;
+  o->cell_size[1] = c.cell_size.y;
+  o->cell_size[2] = c.cell_size.z;
+  o->density = c.density;
+  o->box_size[0] = c.box_size.x;
+  o->box_size[1] = c.box_size.y;
+  o->box_size[2] = c.box_size.z;
+  o->boundary_translation[0] = c.boundary_translation.x;
+  o->boundary_translation[1] = c.boundary_translation.y;
+  o->boundary_translation[2] = c.boundary_translation.z;
+  o->particle_translation[0] = c.particle_translation.x;
+  o->particle_translation[1] = c.particle_translation.y;
+  o->particle_translation[2] = c.particle_translation.z;
+  o->scene = c.scene;
+  o->is_adaptive = c.is_adaptive;
+  copy_str(o->obj_file, sizeof(o->obj_file), c.obj_file);
+  copy_str(o->save_file, sizeof(o->save_file), c.save_file);
+}
+
+SceneConfig from_c(const ggh_scene *s) {
+  SceneConfig c;
+  c.space_size = make_float3(s->space_size[0], s->space_size[1], s->space_size[2]);
+  c.dt = s->dt;
+  c.G = make_float3(s->gravity[0], s->gravity[1], s->gravity[2]);
+  c.sphSpacing = s->sph_spacing;
+  c.initSpacing = s->init_spacing;
+  c.smoothing_radius = s->smoothing_radius;
+  c.cell_length = s->cell_length;
+  c.cell_size = make_int3(s->cell_size[0], s->cell_size[1], s->cell_size[2]);
+  c.density = s->density;
+  c.box_size = make_float3(s->box_size[0], s->box_size[1], s->box_size[2]);
+  c.boundary_translation =
+      make_float3(s->boundary_translation[0], s->boundary_translation[1], s->boundary_translation[2]);
+  c.particle_translation =
+      make_float3(s->particle_translation[0], s->particle_translation[1], s->particle_translation[2]);
+  c.scene = s->scene;
+  c.obj_file = s->obj_file;
+  c.is_adaptive = s->is_adaptive;
+  c.save_file = s->save_file;
+  return c;
+}
+
+int emit(const std::vector<float3> &v, float *out, int max) {
+  if (out) {
+    const size_t k = std::min(v.size(), (size_t)(max > 0 ? max : 0));
+    std::memcpy(out, v.data(), k * sizeof(float3));
+  }
+  return (int)v.size();
+}
+
+std::vector<float3> triples(const float *p, int n) {
+  std::vector<float3> v((size_t)(n > 0 ? n : 0));
+  if (n > 0) std::memcpy(v.data(), p, (size_t)n * sizeof(float3));
+  return v;
+}
+
+std::shared_ptr<GranularParticles> set_of(const ggh_system *h, int which) {
+  switch (which) {
+    case 0: return h->sys->get_particles_non_const();
+    case 1: return h->sys->get_boundaries();
+    case 2: return h->sys->get_upsampled();
+    default: throw std::invalid_argument("set index must be 0, 1 or 2");
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ggh_last_error(void) { return g_err.c_str(); }
+
+int ggh_parse_scene(const char *json_text, ggh_scene *out) {
+  try {
+    if (!json_text || !out) throw std::invalid_argument("null argument");
+    to_c(parseSceneConfig(json_text), out);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_load_scene(const char *path, ggh_scene *out) {
+  try {
+    if (!path || !out) throw std::invalid_argument("null argument");
+    to_c(loadSceneConfig(path), out);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_scene_lattice(const ggh_scene *s, float *out, int max) {
+  try {
+    if (!s) throw std::invalid_argument("null scene");
+    return emit(granular_lattice(from_c(s)), out, max);
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_scene_walls(const ggh_scene *s, float *out, int max) {
+  try {
+    if (!s) throw std::invalid_argument("null scene");
+    return emit(boundary_walls(from_c(s)), out, max);
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_upsampled_cloud(const float *coarse, int n, float radius, int per, float *out, int max) {
+  try {
+    if (n < 0 || per < 0 || (n > 0 && !coarse)) throw std::invalid_argument("bad arguments");
+    return emit(upsampled_cloud(triples(coarse, n), radius, per), out, max);
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+ggh_system *ggh_system_create(const float *pos, int n, const float *boundary, int nb,
+                              const int *boundary_animated, const float *fine, int nf,
+                              const float space_size[3], float cell_length, float dt,
+                              const float gravity[3], const int cell_size[3], float density,
+                              float upsampled_radius, int is_move_boundary, int is_adaptive,
+                              int verbose) {
+  try {
+    if (n < 0 || nb < 0 || nf < 0 || !space_size || !gravity || !cell_size)
+      throw std::invalid_argument("bad arguments");
+    auto gp = std::make_shared<GranularParticles>(triples(pos, n));
+    auto bp = std::make_shared<GranularParticles>(triples(boundary, nb));
+    auto fp = std::make_shared<GranularParticles>(triples(fine, nf));
+    if (boundary_animated && nb > 0)
+      HIP_CALL(hipMemcpy(bp->get_is_animated_ptr(), boundary_animated, sizeof(int) * (size_t)nb,
+                         hipMemcpyHostToDevice));
+    std::unique_ptr<ggh_system> h(new ggh_system);
+    // the constructor runs one step; print nothing for FFI callers unless asked
+    std::streambuf *saved = nullptr;
+    std::ostringstream sink;
+    if (!verbose) saved = std::cout.rdbuf(sink.rdbuf());
+    try {
+      h->sys.reset(new GranularSystem(
+          gp, bp, fp, make_float3(space_size[0], space_size[1], space_size[2]), cell_length, dt,
+          make_float3(gravity[0], gravity[1], gravity[2]),
+          make_int3(cell_size[0], cell_size[1], cell_size[2]), density, upsampled_radius,
+          is_move_boundary != 0, is_adaptive));
+    } catch (...) {
+      if (saved) std::cout.rdbuf(saved);
+      throw;
+    }
+    if (saved) std::cout.rdbuf(saved);
+    h->sys->set_verbose(verbose != 0);
+    return h.release();
+  } catch (const std::exception &e) {
+    fail(e);
+    return nullptr;
+  }
+}
+
+void ggh_system_destroy(ggh_system *h) { delete h; }
+
+int ggh_system_step(ggh_system *h, int nsteps) {
+  try {
+    if (!h || nsteps < 0) throw std::invalid_argument("bad arguments");
+    int r = 0;
+    for (int k = 0; k < nsteps; ++k) r = (int)h->sys->step();
+    return r;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_size(const ggh_system *h, int which) {
+  if (!h) return -1;
+  switch (which) {
+    case 0: return h->sys->size();
+    case 1: return h->sys->boundary_size();
+    case 2: return h->sys->upsampled_size();
+    case 3: return h->sys->total_size();
+    default: return -1;
+  }
+}
+
+int ggh_system_read(const ggh_system *h, int which, int field, void *out, int count) {
+  try {
+    if (!h || !out || count < 0) throw std::invalid_argument("bad arguments");
+    auto s = set_of(h, which);
+    if ((unsigned int)count > s->size()) throw std::invalid_argument("count exceeds set size");
+    const void *src = nullptr;
+    size_t esz = 4;
+    switch (field) {
+      case 0: src = s->get_pos_ptr(); esz = 12; break;
+      case 1: src = s->get_vel_ptr(); esz = 12; break;
+      case 2: src = s->get_mass_ptr(); break;
+      case 3: src = s->get_particle_2_cell(); break;
+      case 4: src = s->get_surface_ptr(); break;
+      case 5: src = s->get_surface_distance_ptr(); break;
+      case 6: src = s->get_adaptive_last_step_ptr(); break;
+      case 7: src = s->get_is_animated_ptr(); break;
+      default: throw std::invalid_argument("unknown field");
+    }
+    if (hipDeviceSynchronize() != hipSuccess) throw std::runtime_error("device error");
+    if (count > 0 && hipMemcpy(out, src, esz * (size_t)count, hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("copy failed");
+    return count;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_translate_animated(ggh_system *h, float dx, float dy, float dz) {
+  try {
+    if (!h) throw std::invalid_argument("null handle");
+    auto b = h->sys->get_boundaries();
+    const unsigned int nb = b->size();
+    std::vector<float3> p(nb);
+    std::vector<int> a(nb);
+    if (nb == 0) return 0;
+    if (hipMemcpy(p.data(), b->get_pos_ptr(), 12 * (size_t)nb, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(a.data(), b->get_is_animated_ptr(), 4 * (size_t)nb, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+      throw std::runtime_error("copy failed");
+    int moved = 0;
+    for (unsigned int i = 0; i < nb; ++i)
+      if (a[i] == 1) {
+        p[i] = make_float3(p[i].x + dx, p[i].y + dy, p[i].z + dz);
+        ++moved;
+      }
+    if (hipMemcpy(b->get_pos_ptr(), p.data(), 12 * (size_t)nb, hipMemcpyHostToDevice) != hipSuccess)
+      throw std::runtime_error("copy failed");
+    return moved;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_frame_times(const ggh_system *h, float *out, int max) {
+  if (!h) return -1;
+  const auto &ft = h->sys->get_frame_times();
+  if (out)
+    for (int i = 0; i < (int)ft.size() && i < max; ++i) out[i] = ft[(size_t)i];
+  return (int)ft.size();
+}
+
+int ggh_system_particle_stats(const ggh_system *h, int *out, int max_rows) {
+  if (!h) return -1;
+  const auto &st = h->sys->get_solver().get_particle_stats();
+  if (out)
+    for (int i = 0; i < (int)st.size() && i < max_rows; ++i) {
+      out[3 * i] = std::get<0>(st[(size_t)i]);
+      out[3 * i + 1] = std::get<1>(st[(size_t)i]);
+      out[3 * i + 2] = std::get<2>(st[(size_t)i]);
+    }
+  return (int)st.size();
+}
+
+int ggh_system_save_stats(ggh_system *h, const char *path) {
+  try {
+    if (!h || !path) throw std::invalid_argument("bad arguments");
+    h->sys->get_solver().save_particle_stats(path);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_set_solver(ggh_system *h, int stab, int solver, float mu_s, float mu_k,
+                          int flags) {
+  try {
+    if (!h) throw std::invalid_argument("null handle");
+    Solver &s = h->sys->get_solver();
+    s.set_iterations(stab, solver);
+    s.set_friction(mu_s, mu_k);
+    s.set_flags(flags);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+}  // extern "C"

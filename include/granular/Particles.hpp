@@ -1,0 +1,51 @@
+// Particle set with positions and velocities (Particles.hpp:7-62 of the
+// reference): built from host positions, velocities start at zero.
+#pragma once
+#include <stdexcept>
+#include <vector>
+
+#include "DArray.hpp"
+#include "Global.hpp"
+
+class Particles {
+ public:
+  explicit Particles(const std::vector<float3> &p)
+      : _pos((unsigned int)p.size()), _vel((unsigned int)p.size()) {
+    if (!p.empty())
+      HIP_CALL(hipMemcpy(_pos.addr(), p.data(), sizeof(float3) * p.size(),
+                         hipMemcpyHostToDevice));
+  }
+
+  Particles(const Particles &) = delete;
+  Particles &operator=(const Particles &) = delete;
+
+  unsigned int size() const { return _pos.length(); }
+  float3 *get_pos_ptr() const { return _pos.addr(); }
+  float3 *get_vel_ptr() const { return _vel.addr(); }
+  const DArray<float3> &get_pos() const { return _pos; }
+
+  void remove_elements(const DArray<int> &removal_flags) {
+    _pos.compact(removal_flags);
+    _vel.compact(removal_flags);
+  }
+
+  void add_elements(const DArray<float3> &pos, const DArray<float3> &vel) {
+    if (pos.length() != vel.length())
+      throw std::runtime_error("Position and velocity arrays must have same length");
+    const unsigned int new_size = _pos.length() + pos.length();
+    if (new_size > _pos.capacity()) {
+      _pos.resize(new_size);
+      _vel.resize(new_size);
+    }
+    _pos.append(pos);
+    _vel.append(vel);
+    if (_pos.length() != _vel.length())
+      throw std::runtime_error("Position and velocity array sizes mismatch after append");
+  }
+
+  virtual ~Particles() noexcept {}
+
+ public:
+  DArray<float3> _pos;
+  DArray<float3> _vel;
+};

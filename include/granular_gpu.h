@@ -278,6 +278,17 @@ int gg_profiler_collect(double *total_ms, int *count, int nstages);
  * particles of sweep k that scanned the full stencil. */
 const void *gg_workspace_view(const gg_workspace *ws, int which);
 
+/* ---- device-array primitives (host DArray class) --------------------------- */
+/* Stable compaction (DArray<T>::compact, DArray.hpp:62-122): copies the
+ * elements i of src with flags[i] != 1, in order, to dst (no aliasing);
+ * elements are elem_words 32-bit words; the kept count is written to the
+ * device int *count_out.  scratch: gg_compact_scratch_bytes(n) bytes. */
+size_t gg_compact_scratch_bytes(long long n);
+int gg_compact(const void *src, void *dst, long long n, int elem_words, const int *flags,
+               int *count_out, void *scratch, size_t scratch_bytes, hipStream_t stream);
+/* words x 32-bit fill (thrust::fill on the reference's arrays). */
+int gg_fill_u32(void *dst, size_t words, uint32_t value, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

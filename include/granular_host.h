@@ -1,0 +1,86 @@
+/* C-ABI of the host C++ API (GranularSystem / Solver / GranularParticles /
+ * DArray / SceneConfig), for callers that cannot use C++ classes (ctypes,
+ * cgo, JNI).  Handles own one GranularSystem each.  Functions return 0 (or a
+ * count) on success and a negative value on error; ggh_last_error() gives
+ * the message of the last failure on the calling thread.
+ *
+ * Reference interface replaced: the GranularSystem class and its accessors
+ * (GranularSystem.hpp:8-56), Solver::save_particle_stats (Solver.cu:1444),
+ * loadSceneConfig (main.cu:123-167) and the scene builders of main.cu
+ * (747-816, 822-1167). */
+#ifndef GRANULAR_HOST_H
+#define GRANULAR_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ggh_system ggh_system;
+
+typedef struct ggh_scene {
+  float space_size[3];
+  float dt;
+  float gravity[3];
+  float sph_spacing;
+  float init_spacing;
+  float smoothing_radius;
+  float cell_length;
+  int cell_size[3];
+  float density;
+  float box_size[3];
+  float boundary_translation[3];
+  float particle_translation[3];
+  int scene;
+  int is_adaptive;
+  char obj_file[256];
+  char save_file[256];
+} ggh_scene;
+
+const char *ggh_last_error(void);
+
+/* Scene JSON (CPU only). */
+int ggh_load_scene(const char *path, ggh_scene *out);
+int ggh_parse_scene(const char *json_text, ggh_scene *out);
+/* Scene builders (CPU only): write up to max xyz triples to out (out may be
+ * NULL to count) and return the total count. */
+int ggh_scene_lattice(const ggh_scene *s, float *out, int max);
+int ggh_scene_walls(const ggh_scene *s, float *out, int max);
+int ggh_upsampled_cloud(const float *coarse, int n, float radius, int per, float *out, int max);
+
+/* System: host arrays are copied to the device; the constructor runs one
+ * step (as the reference's does). */
+ggh_system *ggh_system_create(const float *pos, int n, const float *boundary, int nb,
+                              const int *boundary_animated, const float *fine, int nf,
+                              const float space_size[3], float cell_length, float dt,
+                              const float gravity[3], const int cell_size[3], float density,
+                              float upsampled_radius, int is_move_boundary, int is_adaptive,
+                              int verbose);
+void ggh_system_destroy(ggh_system *h);
+/* Runs nsteps steps; returns the last step()'s value (1). */
+int ggh_system_step(ggh_system *h, int nsteps);
+/* which: 0 granular, 1 boundary, 2 fine, 3 total. */
+int ggh_system_size(const ggh_system *h, int which);
+/* Copies count elements of a field of set `which` (0 granular, 1 boundary,
+ * 2 fine) to host memory.  field: 0 pos (3 floats), 1 vel (3 floats),
+ * 2 mass, 3 particle_2_cell (int), 4 surface (int), 5 surface_distance,
+ * 6 adaptive_last_step (int), 7 is_animated (int). */
+int ggh_system_read(const ggh_system *h, int which, int field, void *out, int count);
+/* Moves the boundary particles flagged as animated by (dx,dy,dz) (a
+ * kinematic boundary; the next step re-sorts a moving boundary). */
+int ggh_system_translate_animated(ggh_system *h, float dx, float dy, float dz);
+int ggh_system_frame_times(const ggh_system *h, float *out, int max);
+/* Rows (merged, split, N before) of the adaptive passes; returns the count. */
+int ggh_system_particle_stats(const ggh_system *h, int *out, int max_rows);
+int ggh_system_save_stats(ggh_system *h, const char *path);
+int ggh_system_set_solver(ggh_system *h, int stab, int solver, float mu_s, float mu_k,
+                          int flags);
+
+/* Device self-test of DArray (compact, append, remove, resize,
+ * verifyArrayOrdering) and GranularParticles (remove_elements,
+ * add_elements) on n elements; 0 when every check passes. */
+int ggh_darray_selftest(int n, char *msg, int msg_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

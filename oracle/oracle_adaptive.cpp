@@ -6,6 +6,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <cstdint>
 #include <vector>
 
 #include "oracle.h"
@@ -18,10 +20,32 @@ static inline V3<float> sub3(V3<float> a, V3<float> b) {
 }
 static inline float dot3(V3<float> a, V3<float> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static inline float len3(V3<float> a) { return std::sqrt(dot3(a, a)); }
-// normalize(v) = v * (1/sqrt(v.v)) (helper_math.h:1325-1329 with the host
-// rsqrtf = 1/sqrtf of helper_math.h:78-81); v = 0 gives NaN.
+// Approximate 1/sqrt(x): the bit-level initial estimate of Lomont, "Fast
+// Inverse Square Root" (2003), refined by two Newton-Raphson steps
+// y <- y (1.5 - (0.5 x) y y); relative error below 5e-6.  Basic IEEE
+// operations only, so the device computes the same bits.
+static inline float rsqrt_approx(float x) {
+  uint32_t i;
+  std::memcpy(&i, &x, 4);
+  i = 0x5f3759dfu - (i >> 1);
+  float y;
+  std::memcpy(&y, &i, 4);
+  const float hx = 0.5f * x;
+  y = y * (1.5f - hx * y * y);
+  y = y * (1.5f - hx * y * y);
+  return y;
+}
+
+// normalize(v) = v * rsqrt(v.v) (helper_math.h:1325-1329).  The reference's
+// surface kernels run on the device, where rsqrtf is an approximation; its
+// error makes normalize(+d e) + normalize(-d' e) nonzero for d != d', so the
+// interior of a regular lattice gets a tiny nonzero cover vector and is not
+// surface.  A correctly rounded 1/sqrt returns exactly +-1 for axis vectors
+// and flags every lattice interior as surface.  rsqrt_approx keeps the
+// approximate behaviour deterministically.  v = 0 gives 0 (the vote then
+// fails exactly as with the NaN of an exact 1/sqrt).
 static inline V3<float> nrm3(V3<float> v) {
-  const float inv = 1.0f / std::sqrt(dot3(v, v));
+  const float inv = rsqrt_approx(dot3(v, v));
   return V3<float>{v.x * inv, v.y * inv, v.z * inv};
 }
 

@@ -53,3 +53,30 @@ def test_host_only_calls():
     # invalid workspace is rejected before any device work
     ws = _lib.gg_workspace()
     assert lib.gg_workspace_init(C.byref(ws), None) == _lib.GG_ERR_INVALID
+
+
+def host_header_functions():
+    txt = (ROOT / "include" / "granular_host.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ggh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_host_library_exports_all_header_symbols():
+    from granular_gpu_amd import host
+    lib = host.load()
+    names = host_header_functions()
+    assert "ggh_system_create" in names and "ggh_load_scene" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", str(host._LIB_PATH)], check=True,
+                         capture_output=True, text=True).stdout
+    assert set(names) <= set(re.findall(r"\bT (ggh_[a-z0-9_]+)", out))
+
+
+def test_cpp_headers_compile_standalone(tmp_path):
+    # every public C++ header of the host API compiles on its own
+    for h in sorted((ROOT / "include" / "granular").glob("*.hpp")):
+        src = tmp_path / f"inc_{h.stem}.cpp"
+        src.write_text(f'#include "granular/{h.name}"\nint main() {{ return 0; }}\n')
+        subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__",
+                        "-I/opt/rocm/include", f"-I{ROOT / 'include'}", str(src)], check=True)

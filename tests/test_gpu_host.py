@@ -1,0 +1,140 @@
+"""GPU tests of the host C++ API (DArray, GranularParticles, Solver,
+GranularSystem) through its C-ABI: container semantics, the reference's
+system semantics (one step in the constructor, step() returns 1, frame
+times, N changes through merge/split) and bit-exact parity of whole system
+steps (granular, fine and adaptive) with the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from granular_gpu_amd import scenes
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("n", [0, 1, 1023, 1024, 1025, 70001])
+def test_darray_selftest(gpu, n):
+    from granular_gpu_amd import host
+    _log(f"DArray self-test n={n}")
+    assert host.darray_selftest(n) == "ok"
+
+
+def _unsorted_keys(keys_sorted, perm):
+    k = np.empty_like(keys_sorted)
+    k[perm] = keys_sorted
+    return k
+
+
+def test_system_plain_with_fine_bitexact(gpu, tmp_path):
+    from granular_gpu_amd import host
+    s = scenes.plain_box(10, 1.0, dt=0.002, g=-9.8)
+    fine = scenes.fine_cloud(s.pos, 6, 0.01, seed=5)
+    _log(f"host system plain n={s.n} fine={fine.shape[0]}")
+    sysm = host.HostSystem.from_scene(s, fine=fine)
+    assert sysm.size("granular") == s.n and sysm.size("fine") == fine.shape[0]
+    assert sysm.size("total") == s.n + s.boundary.shape[0] + fine.shape[0]
+    k = 7
+    assert sysm.step(k) == 1
+    assert sysm.frame_times().shape == (k + 1,)     # the constructor ran one step
+    assert (sysm.frame_times() > 0).all()
+    gp, gv = sysm.read("granular", "pos"), sysm.read("granular", "vel")
+    fp, fv = sysm.read("fine", "pos"), sysm.read("fine", "vel")
+    # oracle: the same pipeline, step by step
+    p = po.params_from(s, 3, 5)
+    bpos, bcs = po.sorted_boundary(s.boundary, s.cell_length, s.cells)
+    cp, cv, cm = s.pos, s.vel, np.ones(s.n, np.float32)
+    op, ov = fine, np.zeros_like(fine)
+    for _ in range(k + 1):
+        _, _, cs = po.grid_build(cp, s.cell_length, s.cells)
+        cp, cv, cm = po.plain_steps_f32(cp, cv, cm, bpos, bcs, p, 1)[:3]
+        keys, perm, _ = po.grid_build(op, s.cell_length, s.cells)
+        keys = _unsorted_keys(keys, perm)
+        op, ov = op[perm], ov[perm]
+        op, ov = po.upsample_plain(op, ov, cp, cv, cs, bpos, bcs, p)
+    assert np.array_equal(gp, cp) and np.array_equal(gv, cv)
+    assert np.array_equal(fp, op) and np.array_equal(fv, ov)
+    assert np.array_equal(sysm.read("fine", "particle_2_cell"), keys)
+    assert np.array_equal(sysm.read("boundary", "pos"), bpos)
+    assert np.array_equal(sysm.read("granular", "mass"), np.ones(s.n, np.float32))
+    sysm.save_stats(tmp_path / "stats.csv")
+    assert (tmp_path / "stats.csv").read_text().splitlines() == [
+        "Frame,MergedParticles,SplitParticles,TotalParticles"]
+    sysm.close()
+
+
+def test_system_solver_options(gpu):
+    # iteration counts and friction set on the system's solver apply from the
+    # next step on (the constructor's step used the reference's 3 + 5, no
+    # friction)
+    from granular_gpu_amd import host
+    s = scenes.plain_box(8, 1.0, dt=0.002, g=-9.8)
+    a = host.HostSystem.from_scene(s)
+    a.set_solver(3, 10, 0.5, 0.4)
+    a.step(3)
+    p = po.params_from(s, 3, 5)
+    bpos, bcs = po.sorted_boundary(s.boundary, s.cell_length, s.cells)
+    cp, cv, cm = po.plain_steps_f32(s.pos, s.vel, s.mass, bpos, bcs, p, 1)[:3]
+    p2 = po.params_from(s, 3, 10, 0.5, 0.4)
+    cp, cv, cm = po.plain_steps_f32(cp, cv, cm, bpos, bcs, p2, 3)[:3]
+    assert np.array_equal(a.read("granular", "pos"), cp)
+    assert np.array_equal(a.read("granular", "vel"), cv)
+    a.close()
+
+
+def test_system_adaptive_bitexact(gpu, tmp_path):
+    from granular_gpu_amd import host
+    s = scenes.plain_box(16, 1.0, dt=0.002, g=-9.8)
+    _log(f"host system adaptive n={s.n}")
+    sysm = host.HostSystem.from_scene(s, is_adaptive=1)
+    k = 79
+    sysm.step(k)
+    n = sysm.size("granular")
+    st = po.AdaptiveState(s.pos, s.vel, s.mass, cap=2 * s.n)
+    p = po.params_from(s, 3, 5)
+    bpos, bcs = po.sorted_boundary(s.boundary, s.cell_length, s.cells)
+    stats = po.adaptive_steps(st, bpos, bcs, p, k + 1)
+    _log(f"merged {stats[:, 0].sum()} split {stats[:, 1].sum()} final n {st.n} (gpu {n})")
+    assert stats[:, 0].sum() > 0, "no merge happened: the test scene is too small"
+    assert n == st.n < s.n
+    assert np.array_equal(sysm.particle_stats(), stats)
+    assert np.array_equal(sysm.read("granular", "pos"), st.pos[:n])
+    assert np.array_equal(sysm.read("granular", "vel"), st.vel[:n])
+    assert np.array_equal(sysm.read("granular", "mass"), st.mass[:n])
+    assert np.array_equal(sysm.read("granular", "adaptive_last_step"), st.adl[:n])
+    sysm.save_stats(tmp_path / "stats.csv")
+    rows = (tmp_path / "stats.csv").read_text().splitlines()
+    assert rows[0] == "Frame,MergedParticles,SplitParticles,TotalParticles"
+    assert rows[1:] == [f"{i},{a},{b},{c}" for i, (a, b, c) in enumerate(stats)]
+    sysm.close()
+
+
+def test_system_moving_boundary(gpu):
+    # a kinematic boundary: the animated wall particles move between steps
+    # and the next step re-sorts the boundary (is_move_boundary)
+    from granular_gpu_amd import host
+    s = scenes.plain_box(8, 1.0, dt=0.002, g=-9.8)
+    anim = np.zeros(s.boundary.shape[0], np.int32)
+    anim[s.boundary[:, 1] < 0.02] = 1            # the floor
+    sysm = host.HostSystem.from_scene(s, is_move_boundary=True, boundary_animated=anim)
+    sysm.step(2)
+    gp0, gv0 = sysm.read("granular", "pos"), sysm.read("granular", "vel")
+    b0 = sysm.read("boundary", "pos")
+    a0 = sysm.read("boundary", "is_animated")
+    assert sysm.translate_animated((0.0, 0.03, 0.0)) == int(anim.sum())
+    sysm.step(1)
+    moved = b0.copy()
+    moved[a0 == 1] += np.float32([0.0, 0.03, 0.0])
+    bpos, bcs = po.sorted_boundary(moved, s.cell_length, s.cells)
+    assert np.array_equal(sysm.read("boundary", "pos"), bpos)
+    p = po.params_from(s, 3, 5)
+    cp, cv, _ = po.plain_steps_f32(gp0, gv0, np.ones(s.n, np.float32), bpos, bcs, p, 1)[:3]
+    assert np.array_equal(sysm.read("granular", "pos"), cp)
+    assert np.array_equal(sysm.read("granular", "vel"), cv)
+    sysm.close()

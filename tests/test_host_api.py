@@ -1,0 +1,110 @@
+"""Host C++ API, CPU side: the scene JSON reader against Python's json
+module, the wall and lattice builders against the Python scene generator
+and closed forms, the fine-particle cloud against the standard library's
+minstd_rand0 + generate_canonical arithmetic, and error behaviour.  No GPU
+calls."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from granular_gpu_amd import host, scenes
+
+SCENES = sorted((Path(__file__).parent / "golden" / "scenes").glob("*.json"))
+f32 = np.float32
+
+
+@pytest.mark.parametrize("path", SCENES, ids=lambda p: p.stem)
+def test_scene_json_matches_python(path):
+    s = host.load_scene(path)
+    j = json.loads(path.read_text())
+    v3 = lambda k: [f32(j[k][a]) for a in "xyz"]
+    assert list(s.space_size) == v3("space_size")
+    assert list(s.gravity) == v3("gravity")
+    assert list(s.box_size) == v3("box_size")
+    assert list(s.boundary_translation) == v3("boundary_translation")
+    assert list(s.particle_translation) == v3("particle_translation")
+    for k, a in (("dt", "dt"), ("sph_spacing", "sph_spacing"), ("init_spacing", "init_spacing"),
+                 ("smoothing_radius", "smoothing_radius"), ("cell_length", "cell_length"),
+                 ("density", "density")):
+        assert f32(getattr(s, a)) == f32(j[k]), k
+    assert s.scene == j["scene"] and s.is_adaptive == j["is_adaptive"]
+    assert s.obj_file.decode() == j["obj_file"] and s.save_file.decode() == j["save_file"]
+    L = f32(j["cell_length"])
+    want = [int(np.ceil(f32(j["space_size"][a]) / L)) for a in "xyz"]
+    assert list(s.cell_size) == want
+
+
+def test_scene_json_errors():
+    with pytest.raises(host.HostError, match="Unable to open"):
+        host.load_scene("/nonexistent/scene.json")
+    good = json.loads(SCENES[0].read_text())
+    del good["dt"]
+    with pytest.raises(host.HostError, match="missing key dt"):
+        host.parse_scene(json.dumps(good))
+    with pytest.raises(host.HostError):
+        host.parse_scene('{"space_size": {"x": 1,')
+
+
+def test_box_scene_counts():
+    # the reference box scene: 50 x 50 x 40 lattice, 51,088 wall particles
+    s = host.load_scene(SCENES[[p.stem for p in SCENES].index("box")])
+    assert host.scene_lattice(s).shape == (100000, 3)
+    assert host.scene_walls(s).shape == (51088, 3)
+
+
+@pytest.mark.parametrize("name,recipe", [("box", "box"), ("piling", "piling"),
+                                         ("corner", "corner"), ("box_opening_up", "box_opening")])
+def test_walls_match_python_recipe(name, recipe):
+    s = host.load_scene(Path(__file__).parent / "golden" / "scenes" / f"{name}.json")
+    w = host.scene_walls(s)
+    pw = scenes.box_walls(tuple(s.space_size), s.cell_length, recipe,
+                          tuple(s.boundary_translation))
+    assert np.array_equal(w, pw)
+    # every wall particle lies on a face of the scaled box
+    S = np.asarray(s.space_size, np.float32)
+    lo = f32(0.005) * S + np.asarray(s.boundary_translation, np.float32)
+    hi = lo + f32(0.99) * S
+    on_face = np.isclose(w, lo, atol=1e-6) | np.isclose(w, hi, atol=1e-6)
+    assert on_face.any(axis=1).all()
+
+
+@pytest.mark.parametrize("name", ["excavator", "funnel"])
+def test_walls_need_geometry(name):
+    s = host.load_scene(Path(__file__).parent / "golden" / "scenes" / f"{name}.json")
+    with pytest.raises(host.HostError, match="external geometry"):
+        host.scene_walls(s)
+
+
+def test_lattice_formula():
+    s = host.load_scene(SCENES[[p.stem for p in SCENES].index("box")])
+    L = host.scene_lattice(s)
+    sp, t = f32(s.init_spacing), np.asarray(s.particle_translation, np.float32)
+    nx, ny, nz = (int(v) for v in s.box_size)
+    i, j, k = 3, 7, 11
+    p = L[(i * ny + j) * nz + k]
+    want = np.array([f32(0.27) + sp * f32(j), f32(0.13) + sp * f32(i),
+                     f32(0.17) + sp * f32(k)], np.float32) + t
+    assert np.array_equal(p, want)
+
+
+def _minstd_canonical(count):
+    """minstd_rand0 (x <- 16807 x mod 2^31-1, seed 1) mapped to [0,1) as
+    float(x - 1) / float(2^31 - 2) in float arithmetic."""
+    x, out = 1, []
+    for _ in range(count):
+        x = (16807 * x) % 2147483647
+        out.append(f32(x - 1) / f32(2147483646.0))
+    return np.array(out, np.float32)
+
+
+def test_upsampled_cloud_generator():
+    c = np.float32([[0.5, 0.6, 0.7], [0.1, 0.2, 0.3]])
+    r, per = 0.01, 4
+    got = host.upsampled_cloud(c, r, per)
+    u = _minstd_canonical(2 * per * 3).reshape(2, per, 3)
+    off = u * (f32(r) - f32(-r)) + f32(-r)
+    want = (c[:, None, :] + off).reshape(-1, 3)
+    assert np.array_equal(got, want)
+    assert np.all(np.abs(got.reshape(2, per, 3) - c[:, None]) <= r)
