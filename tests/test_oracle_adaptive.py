@@ -248,3 +248,28 @@ def test_adaptive_steps_conserve_mass():
     assert abs(total - total0) / total0 < 1e-4
     assert st.n == s.n - stats[:, 0].sum() + stats[:, 1].sum()
     assert np.isfinite(st.pos[: st.n]).all()
+
+
+def test_regular_lattice_interior_is_not_surface(lib):
+    # normalize uses an approximate rsqrt (as the reference's device rsqrtf
+    # is): for a regular lattice the +-axis unit vectors of an interior
+    # particle then do not cancel exactly, the cover vector is tiny but
+    # nonzero and some axis neighbour lies within 60 degrees of it.  With a
+    # correctly rounded 1/sqrt they cancel exactly and every particle would
+    # be flagged surface (NaN vote).
+    side = 6
+    pos = np.array([[0.3 + 0.032 * i, 0.3 + 0.032 * j, 0.3 + 0.032 * k]
+                    for i in range(side) for j in range(side) for k in range(side)], np.float32)
+    s = _scene(pos)
+    spos, cs, perm = _sorted(s)
+    cover, surf = _surface(lib, s, spos, cs)
+    q = np.rint((spos - 0.3) / 0.032).astype(int)
+    interior = np.all((q > 0) & (q < side - 1), axis=1)
+    shell = ~interior
+    assert interior.sum() == (side - 2) ** 3
+    assert surf[shell].all()                       # n <= 5 or an open side
+    assert surf[interior].mean() < 0.5, surf[interior].mean()
+    nz = np.any(cover[interior] != 0, axis=1)
+    assert nz.mean() > 0.5
+    # cover vectors of interior particles are rounding-sized
+    assert np.abs(cover[interior]).max() < 1e-4
