@@ -35,7 +35,8 @@ class gg_params(C.Structure):
         ("mu_s", C.c_float), ("mu_k", C.c_float), ("boundary_radius", C.c_float),
         ("contact_radius_min", C.c_float), ("sleep_speed", C.c_float),
         ("damp_speed", C.c_float), ("damping", C.c_float), ("adaptive_damping", C.c_float),
-        ("skin", C.c_float), ("flags", C.c_int),
+        ("skin", C.c_float), ("flags", C.c_int), ("sweep_block", C.c_int),
+        ("nlist_block", C.c_int),
     ]
 
 
@@ -67,6 +68,16 @@ class gg_workspace(C.Structure):
         ("boundary_capacity", C.c_int), ("grid", gg_grid), ("max_sweeps", C.c_int),
         ("gmax", C.c_int), ("bmax", C.c_int),
     ]
+
+
+class gg_batch_scene(C.Structure):
+    _fields_ = [("set", gg_particles), ("cell_start", C.c_void_p), ("boundary", gg_particles),
+                ("bcell_start", C.c_void_p), ("params", gg_params), ("ws", gg_workspace)]
+
+
+class gg_batch_config(C.Structure):
+    _fields_ = [("branches", C.c_int), ("sweep_block", C.c_int), ("nlist_block", C.c_int),
+                ("steps_per_launch", C.c_int)]
 
 
 P = C.POINTER
@@ -104,6 +115,16 @@ _PROTOS = {
     "gg_upsampled_sort": (C.c_int, [P(gg_particles), VP, P(gg_workspace), VP]),
     "gg_upsample_plain": (C.c_int, [P(gg_particles), P(gg_particles), VP, P(gg_particles), VP,
                                     gg_grid, P(gg_upsample_params), C.c_int, VP]),
+    "gg_batch_create": (C.c_int, [P(gg_batch_scene), C.c_int, P(gg_batch_config), P(VP)]),
+    "gg_batch_step": (C.c_int, [VP, VP]),
+    "gg_batch_step_eager": (C.c_int, [VP, VP]),
+    "gg_batch_get_config": (C.c_int, [VP, P(gg_batch_config)]),
+    "gg_batch_set_config": (C.c_int, [VP, P(gg_batch_config)]),
+    "gg_batch_autotune": (C.c_int, [VP, C.c_int, VP, P(gg_batch_config), P(C.c_float)]),
+    "gg_batch_destroy": (None, [VP]),
+    "gg_compact_scratch_bytes": (C.c_size_t, [C.c_longlong]),
+    "gg_compact": (C.c_int, [VP, VP, C.c_longlong, C.c_int, VP, VP, VP, C.c_size_t, VP]),
+    "gg_fill_u32": (C.c_int, [VP, C.c_size_t, C.c_uint32, VP]),
     "gg_profiler_enable": (C.c_int, [C.c_int]),
     "gg_profiler_collect": (C.c_int, [P(C.c_double), P(C.c_int), C.c_int]),
 }

@@ -22,9 +22,16 @@ Prof &prof() {
 }
 }  // namespace
 
+// Stages enqueued while a stream is being captured into a graph are not
+// timed (an event record would become a graph node).
+static bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 void prof_begin(int stage, hipStream_t st) {
   Prof &p = prof();
-  if (!p.on || p.next + 2 > (int)p.ev.size()) {
+  if (!p.on || p.next + 2 > (int)p.ev.size() || capturing(st)) {
     p.open_stage = -1;
     return;
   }

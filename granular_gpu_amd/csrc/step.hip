@@ -70,7 +70,8 @@ __global__ void __launch_bounds__(kBlock)
 
 // Candidate lists: particles of the stencil of trunc(x*/L) with
 // d < rc + skin, in scan order (cells ascending, index ascending).
-__global__ void __launch_bounds__(kBlock)
+template <int BS>
+__global__ void __launch_bounds__(BS)
     k_nlist(int n, const float4 *__restrict__ x, const float4 *__restrict__ off,
             const float4 *__restrict__ bnd4, const int *__restrict__ cs,
             const int *__restrict__ bcs, DevGrid g, SweepConst sc, float skin,
@@ -211,8 +212,8 @@ __device__ __forceinline__ void full_stencil(const SweepArgs &a, int i, const fl
 // One sweep.  Particles whose candidate list is provably exact for this
 // sweep use it; the others scan the full stencil.  Both visit contacts in
 // the same order, so the result does not depend on which path ran.
-template <bool FRIC, bool LAST_STAB>
-__global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs a) {
+template <bool FRIC, bool LAST_STAB, int BS>
+__global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
   const int i = xcd_block(blockIdx.x, gridDim.x, a.remap) * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const DevGrid &g = a.g;
@@ -316,11 +317,16 @@ __global__ void __launch_bounds__(kBlock)
 
 static inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// Threads per block of the list build and the sweeps (0 = 256).
+static inline int block_of(int b) { return b == 0 ? 256 : b; }
+static inline bool block_ok(int b) { return b == 0 || b == 64 || b == 128 || b == 256 || b == 512; }
+
 static int check_params(const gg_params *p, const gg_workspace *ws) {
   if (!p) return GG_ERR_INVALID;
   if (p->stab_iters < 0 || p->solver_iters < 0) return GG_ERR_INVALID;
   if (p->stab_iters + p->solver_iters > ws->max_sweeps) return GG_ERR_CAPACITY;
   if (!(p->dt > 0.0f)) return GG_ERR_INVALID;
+  if (!block_ok(p->sweep_block) || !block_ok(p->nlist_block)) return GG_ERR_INVALID;
   return GG_OK;
 }
 
@@ -339,6 +345,27 @@ static SweepConst sweep_const(const gg_params *p, const gg_workspace *ws) {
   sc.bmax = ws->bmax;
   sc.cap = ws->capacity + 1;
   return sc;
+}
+
+template <int BS>
+static int launch_nlist_bs(int n, const float4 *x, const float4 *off, const float4 *bnd4,
+                           const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
+                           int4 *meta, int *glist, int *blist, int remap, hipStream_t st) {
+  k_nlist<BS><<<(n + BS - 1) / BS, BS, 0, st>>>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta,
+                                                 glist, blist, remap);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+
+static int launch_nlist(int bs, int n, const float4 *x, const float4 *off, const float4 *bnd4,
+                        const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
+                        int4 *meta, int *glist, int *blist, int remap, hipStream_t st) {
+  switch (bs) {
+    case 64: return launch_nlist_bs<64>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+    case 128: return launch_nlist_bs<128>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+    case 512: return launch_nlist_bs<512>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+    default: return launch_nlist_bs<256>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+  }
 }
 
 // permute (perm may be null = identity) + predict + lists.
@@ -365,11 +392,10 @@ static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, cons
   prof_end(GG_STAGE_PREDICT, st);
   if (lists_on(p, ws) && n > 0) {
     prof_begin(GG_STAGE_NLIST, st);
-    k_nlist<<<blocks_for(n), kBlock, 0, st>>>(
-        n, at<float4>(ws, L.x0), at<float4>(ws, L.off0), at<float4>(ws, L.bnd4), cs, bcs,
-        dev_grid(ws->grid), sweep_const(p, ws), p->skin, at<int4>(ws, L.meta),
-        at<int>(ws, L.glist), at<int>(ws, L.blist), (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1);
-    GG_LAUNCH_CHECK();
+    GG_TRY(launch_nlist(block_of(p->nlist_block), n, at<float4>(ws, L.x0), at<float4>(ws, L.off0),
+                        at<float4>(ws, L.bnd4), cs, bcs, dev_grid(ws->grid), sweep_const(p, ws),
+                        p->skin, at<int4>(ws, L.meta), at<int>(ws, L.glist),
+                        at<int>(ws, L.blist), (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1, st));
     prof_end(GG_STAGE_NLIST, st);
   }
   (void)nbnd;
@@ -377,8 +403,14 @@ static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, cons
 }
 
 template <bool FRIC, bool LAST>
-static int launch_sweep(const SweepArgs &a, int nblocks, hipStream_t st) {
-  k_sweep<FRIC, LAST><<<nblocks, kBlock, 0, st>>>(a);
+static int launch_sweep(const SweepArgs &a, int bs, hipStream_t st) {
+  const int n = a.n;
+  switch (bs) {
+    case 64: k_sweep<FRIC, LAST, 64><<<(n + 63) / 64, 64, 0, st>>>(a); break;
+    case 128: k_sweep<FRIC, LAST, 128><<<(n + 127) / 128, 128, 0, st>>>(a); break;
+    case 512: k_sweep<FRIC, LAST, 512><<<(n + 511) / 512, 512, 0, st>>>(a); break;
+    default: k_sweep<FRIC, LAST, 256><<<(n + 255) / 256, 256, 0, st>>>(a); break;
+  }
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
@@ -411,16 +443,16 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   a.sc = sweep_const(p, ws);
   a.use_lists = lists_on(p, ws) ? 1 : 0;
   a.remap = (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1;
-  const int nb = blocks_for(n);
+  const int bs = block_of(p->sweep_block);
   const int stage = k < S ? GG_STAGE_STAB : GG_STAGE_SWEEP;
   prof_begin(stage, st);
   int rc;
   if (last_stab)
-    rc = launch_sweep<false, true>(a, nb, st);
+    rc = launch_sweep<false, true>(a, bs, st);
   else if (fric)
-    rc = launch_sweep<true, false>(a, nb, st);
+    rc = launch_sweep<true, false>(a, bs, st);
   else
-    rc = launch_sweep<false, false>(a, nb, st);
+    rc = launch_sweep<false, false>(a, bs, st);
   if (rc != GG_OK) return rc;
   prof_end(stage, st);
   return GG_OK;

@@ -249,3 +249,69 @@ class FineSet:
     def host(self):
         torch.cuda.synchronize()
         return self.pos.cpu().numpy().copy(), self.vel.cpu().numpy().copy()
+
+
+class BatchSim:
+    """Independent plain scenes advanced together by one hipGraph launch
+    (gg_batch_*).  `sims` are PlainSim objects; their arrays, cell tables
+    and workspaces are used in place."""
+
+    def __init__(self, sims, branches=None, sweep_block=0, nlist_block=0, steps_per_launch=1):
+        lib = load()
+        self.lib, self.sims = lib, list(sims)
+        arr = (_lib.gg_batch_scene * len(self.sims))()
+        for k, s in enumerate(self.sims):
+            arr[k].set = s.set
+            arr[k].cell_start = _ptr(s.cs)
+            arr[k].boundary = s.bset
+            arr[k].bcell_start = _ptr(s.bcs)
+            arr[k].params = s.params
+            arr[k].ws = s.ws
+        cfg = _lib.gg_batch_config(branches or min(4, len(self.sims)), sweep_block, nlist_block,
+                                   steps_per_launch)
+        h = C.c_void_p()
+        check("gg_batch_create", lib.gg_batch_create(arr, len(self.sims), C.byref(cfg),
+                                                     C.byref(h)))
+        self.h = h
+        self._arr = arr
+
+    def step(self, launches=1):
+        for _ in range(launches):
+            check("gg_batch_step", self.lib.gg_batch_step(self.h, _stream()))
+
+    def step_eager(self, launches=1):
+        for _ in range(launches):
+            check("gg_batch_step_eager", self.lib.gg_batch_step_eager(self.h, _stream()))
+
+    def config(self):
+        c = _lib.gg_batch_config()
+        check("gg_batch_get_config", self.lib.gg_batch_get_config(self.h, C.byref(c)))
+        return dict(branches=c.branches, sweep_block=c.sweep_block, nlist_block=c.nlist_block,
+                    steps_per_launch=c.steps_per_launch)
+
+    def set_config(self, **kw):
+        c = self.config()
+        c.update(kw)
+        cfg = _lib.gg_batch_config(c["branches"], c["sweep_block"], c["nlist_block"],
+                                   c["steps_per_launch"])
+        check("gg_batch_set_config", self.lib.gg_batch_set_config(self.h, C.byref(cfg)))
+
+    def autotune(self, trials=2):
+        best = _lib.gg_batch_config()
+        ms = C.c_float()
+        check("gg_batch_autotune", self.lib.gg_batch_autotune(self.h, trials, _stream(),
+                                                              C.byref(best), C.byref(ms)))
+        return dict(branches=best.branches, sweep_block=best.sweep_block,
+                    nlist_block=best.nlist_block, steps_per_launch=best.steps_per_launch,
+                    ms_per_launch=ms.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -162,3 +162,25 @@ def fine_cloud(coarse: np.ndarray, per: int, radius: float, seed: int = 7) -> np
     u = uniform01(seed, n * per * 3).reshape(n, per, 3)
     off = ((2.0 * u - 1.0) * float(radius)).astype(np.float32)
     return (c[:, None, :] + off).reshape(n * per, 3).astype(np.float32)
+
+
+def batch_box_sizes(count: int = 64, nmin: int = 4096, nmax: int = 262144, seed: int = 4):
+    """Per-scene particle counts log-uniform in [nmin, nmax] (SplitMix64,
+    seeded), rounded to lattice sides, and per-scene friction (mu_s uniform
+    in [0.3, 0.7], mu_k = 0.8 mu_s).  Returns [(n_side, mu_s, mu_k)]."""
+    u = uniform01(seed, 2 * count).reshape(count, 2)
+    lo, hi = math.log(nmin), math.log(nmax)
+    out = []
+    for a, b in u:
+        n = math.exp(lo + (hi - lo) * float(a))
+        side = max(2, int(round(n ** (1.0 / 3.0))))
+        mu_s = 0.3 + 0.4 * float(b)
+        out.append((side, mu_s, 0.8 * mu_s))
+    return out
+
+
+def batch_box(n_side: int, seed: int = 1) -> Scene:
+    """A plain box scene sized for n_side^3 particles (r = 0.01): box edge
+    max(0.024 n_side, 0.02 (n_side - 1) + 0.25), dt = 0.002, g = -9.8."""
+    box = max(0.024 * n_side, 0.02 * (n_side - 1) + 0.25)
+    return plain_box(n_side, float(np.float32(box)), seed=seed, dt=0.002, g=-9.8)

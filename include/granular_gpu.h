@@ -71,6 +71,10 @@ typedef struct gg_params {
   float adaptive_damping;   /* 0.98  (Solver.cu:242)                        */
   float skin;               /* neighbour-list skin; <= 0 disables lists     */
   int flags;                /* GG_FLAG_*                                    */
+  int sweep_block;          /* threads per block of the contact sweeps:
+                               64, 128, 256 or 512; 0 = 256 (autotuned by
+                               gg_batch_autotune)                           */
+  int nlist_block;          /* same for the neighbour-list build            */
 } gg_params;
 
 #define GG_FLAG_NO_NLIST 1     /* always scan the 27-cell stencil */
@@ -277,6 +281,45 @@ int gg_profiler_collect(double *total_ms, int *count, int nstages);
  * stabilisation, 7 dt*v after gravity, 8 int counters: [1 + k % 62] =
  * particles of sweep k that scanned the full stencil. */
 const void *gg_workspace_view(const gg_workspace *ws, int which);
+
+/* ---- batched scenes -------------------------------------------------------- */
+/* Many independent plain scenes advanced together by one hipGraph launch
+ * (no reference counterpart: the specification's batched-scenes
+ * configuration).  Each scene brings its own arrays, cell tables, params
+ * (per-scene friction) and workspace; N is fixed (no merge/split).  A batch
+ * of one scene is the captured fixed-N step.  The batch object (graph,
+ * streams, events) is allocated by the library; gg_batch_autotune also
+ * allocates a temporary snapshot of the scenes. */
+typedef struct gg_batch gg_batch;
+typedef struct gg_batch_scene {
+  gg_particles set;
+  int *cell_start;          /* C+2 */
+  gg_particles boundary;    /* built by gg_boundary_build into ws */
+  const int *bcell_start;
+  gg_params params;
+  gg_workspace ws;
+} gg_batch_scene;
+typedef struct gg_batch_config {
+  int branches;          /* parallel graph branches (1..16, <= scenes)     */
+  int sweep_block;       /* 0 = each scene's own params value, else 64..512 */
+  int nlist_block;       /* same for the neighbour-list build               */
+  int steps_per_launch;  /* steps of every scene per graph launch (1..1000) */
+} gg_batch_config;
+/* cfg NULL: min(4, nscenes) branches, scene block sizes, 1 step per launch. */
+int gg_batch_create(const gg_batch_scene *scenes, int nscenes, const gg_batch_config *cfg,
+                    gg_batch **out);
+/* One graph launch on `stream`: steps_per_launch steps of every scene. */
+int gg_batch_step(gg_batch *b, hipStream_t stream);
+/* The same steps launched kernel by kernel, scene after scene. */
+int gg_batch_step_eager(gg_batch *b, hipStream_t stream);
+int gg_batch_get_config(const gg_batch *b, gg_batch_config *cfg);
+int gg_batch_set_config(gg_batch *b, const gg_batch_config *cfg);
+/* Times `trials` launches of every candidate (branches x sweep block x list
+ * block), restores the scenes after each, keeps the fastest.  Scene state
+ * is unchanged on return; results never depend on the configuration. */
+int gg_batch_autotune(gg_batch *b, int trials, hipStream_t stream, gg_batch_config *best,
+                      float *best_ms);
+void gg_batch_destroy(gg_batch *b);
 
 /* ---- device-array primitives (host DArray class) --------------------------- */
 /* Stable compaction (DArray<T>::compact, DArray.hpp:62-122): copies the
