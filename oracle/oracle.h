@@ -167,6 +167,31 @@ float oracle_expf(float x);
 void oracle_set_threads(int t);
 int oracle_get_threads(void);
 
+/* ---- learned upsampler (see oracle_mlp.cpp for the definitions) ---------- */
+typedef struct oracle_mlp_cfg {
+  float radius;          /* coarse radius R: position feature scale, template size */
+  float vscale;          /* velocity feature scale V (m/s)                          */
+  float guard_boundary;  /* boundary particle closer than this: fallback            */
+  float guard_floor;     /* y below this: fallback                                  */
+  float guard_side;      /* x or z below this: fallback                             */
+} oracle_mlp_cfg;
+float oracle_bf16_round(float x);
+int oracle_mlp_features(int n, const float *pos, const float *vel, const int *cs, int nb,
+                        const float *bpos, const int *bcs, const oracle_params *p,
+                        const oracle_mlp_cfg *c, float *feat, int *flags, int *knn);
+int oracle_mlp_teacher(int n, const float *pos, const float *vel, const int *cs,
+                       const float *bpos, const int *bcs, const oracle_params *p,
+                       const oracle_upsample_params *u, const oracle_mlp_cfg *c, float *y,
+                       float *fine);
+int oracle_mlp_positions(int n, const float *pos, const float *vel, const float *y,
+                         const oracle_mlp_cfg *c, float dt, float *fine);
+int oracle_mlp_forward(int B, const float *x, const float *W, const float *b, float *y,
+                       int exact);
+double oracle_mlp_backward(int B, const float *x, const float *W, const float *b,
+                           const float *target, float *dW, float *db, int exact);
+int oracle_mlp_adam(int count, float *w, float *m, float *v, const float *g, float lr,
+                    float beta1, float beta2, float eps, float c1, float c2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,6 +1,7 @@
 // Declarations shared between the oracle's translation units -- TEST
 // INFRASTRUCTURE ONLY.
 #pragma once
+#include "oracle.h"
 #include "oracle_common.hpp"
 
 namespace orc {
@@ -13,6 +14,9 @@ void merge_mark(int n, const V3<float> *pos, const float *mass, const V3<float> 
                 const int *cs, const Grid &g, float max_mass, int *cand_out, float *dens_out);
 void merge_blend(int n, float *merge, float *mass, V3<float> *vel, int *mcount, int *remove,
                  const V3<float> *mv, int blend, int *adl);
+void upsample_one(V3<float> pi, V3<float> vi, const V3<float> *cpos, const V3<float> *cvel,
+                  const int *cs, const V3<float> *bpos, const int *bcs, const Grid &g,
+                  const oracle_upsample_params *u, V3<float> &np_out, V3<float> &v_out);
 }  // namespace orc
 
 extern "C" int oracle_split_compact(int n, int cap, float *pos, float *vel, float *mass,

@@ -247,3 +247,117 @@ def upsample_plain(fpos, fvel, cpos, cvel, cs, bpos_sorted, bcs, params, u=None,
                                   np.ascontiguousarray(bpos_sorted, np.float32).reshape(-1), bcs,
                                   C.byref(params), C.byref(u))
     return fp_, fv_
+
+
+class oracle_mlp_cfg(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("radius", "vscale", "guard_boundary", "guard_floor",
+                                         "guard_side")]
+
+
+MLP_W = 49152     # bf16 weights of the 4 layers, [n_out][k_in] each, layer 4 padded to 32 rows
+MLP_B = 416       # fp32 biases
+MLP_WOFF = (0, 12288, 28672, 45056)
+MLP_BOFF = (0, 128, 256, 384)
+MLP_DIMS = ((96, 128), (128, 128), (128, 128), (128, 32))
+
+
+def mlp_cfg(radius=0.02, vscale=1.0, guard_boundary=0.05, guard_floor=0.08, guard_side=0.05):
+    return oracle_mlp_cfg(radius, vscale, guard_boundary, guard_floor, guard_side)
+
+
+def _mlp_protos(lib):
+    f32 = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    P = C.POINTER(oracle_params)
+    lib.oracle_bf16_round.argtypes = [C.c_float]
+    lib.oracle_bf16_round.restype = C.c_float
+    lib.oracle_mlp_features.argtypes = [C.c_int, f32, f32, i32, C.c_int, f32, i32, P,
+                                        C.POINTER(oracle_mlp_cfg), f32, i32, C.c_void_p]
+    lib.oracle_mlp_teacher.argtypes = [C.c_int, f32, f32, i32, f32, i32, P,
+                                       C.POINTER(oracle_upsample_params),
+                                       C.POINTER(oracle_mlp_cfg), C.c_void_p, C.c_void_p]
+    lib.oracle_mlp_positions.argtypes = [C.c_int, f32, f32, f32, C.POINTER(oracle_mlp_cfg),
+                                         C.c_float, f32]
+    lib.oracle_mlp_forward.argtypes = [C.c_int, f32, f32, f32, f32, C.c_int]
+    lib.oracle_mlp_backward.argtypes = [C.c_int, f32, f32, f32, f32, f32, f32, C.c_int]
+    lib.oracle_mlp_backward.restype = C.c_double
+    lib.oracle_mlp_adam.argtypes = [C.c_int, f32, f32, f32, f32, C.c_float, C.c_float,
+                                    C.c_float, C.c_float, C.c_float, C.c_float]
+
+
+def _f(a, shape=None):
+    a = np.ascontiguousarray(a, np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+def mlp_features(pos, vel, cs, bpos_sorted, bcs, params, cfg):
+    lib = load()
+    _mlp_protos(lib)
+    n = pos.shape[0]
+    feat = np.zeros((n, 96), np.float32)
+    flags = np.zeros(n, np.int32)
+    knn = np.zeros((n, 16), np.int32)
+    lib.oracle_mlp_features(n, _f(pos).reshape(-1), _f(vel).reshape(-1), cs,
+                            bpos_sorted.shape[0], _f(bpos_sorted).reshape(-1), bcs,
+                            C.byref(params), C.byref(cfg), feat.reshape(-1), flags,
+                            knn.ctypes.data)
+    return feat, flags, knn
+
+
+def mlp_teacher(pos, vel, cs, bpos_sorted, bcs, params, cfg, u=None):
+    lib = load()
+    _mlp_protos(lib)
+    n = pos.shape[0]
+    u = u or upsample_params()
+    y = np.zeros((n, 24), np.float32)
+    fine = np.zeros((n, 24), np.float32)
+    lib.oracle_mlp_teacher(n, _f(pos).reshape(-1), _f(vel).reshape(-1), cs,
+                           _f(bpos_sorted).reshape(-1), bcs, C.byref(params), C.byref(u),
+                           C.byref(cfg), y.ctypes.data, fine.ctypes.data)
+    return y, fine
+
+
+def mlp_positions(pos, vel, y, cfg, dt=0.002):
+    lib = load()
+    _mlp_protos(lib)
+    n = pos.shape[0]
+    fine = np.zeros((n, 24), np.float32)
+    lib.oracle_mlp_positions(n, _f(pos).reshape(-1), _f(vel).reshape(-1), _f(y).reshape(-1),
+                             C.byref(cfg), dt, fine.reshape(-1))
+    return fine
+
+
+def mlp_forward(x, W, b, exact=0):
+    lib = load()
+    _mlp_protos(lib)
+    B = x.shape[0]
+    y = np.zeros((B, 24), np.float32)
+    lib.oracle_mlp_forward(B, _f(x).reshape(-1), _f(W), _f(b), y.reshape(-1), exact)
+    return y
+
+
+def mlp_backward(x, W, b, t, exact=0):
+    lib = load()
+    _mlp_protos(lib)
+    B = x.shape[0]
+    dW = np.zeros(MLP_W, np.float32)
+    db = np.zeros(MLP_B, np.float32)
+    loss = lib.oracle_mlp_backward(B, _f(x).reshape(-1), _f(W), _f(b), _f(t).reshape(-1), dW,
+                                   db, exact)
+    return loss, dW, db
+
+
+def mlp_adam(w, m, v, g, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
+    lib = load()
+    _mlp_protos(lib)
+    c1 = float(np.float32(1.0 / (1.0 - beta1 ** step)))
+    c2 = float(np.float32(1.0 / (1.0 - beta2 ** step)))
+    w, m, v = (np.ascontiguousarray(a, np.float32).copy() for a in (w, m, v))
+    lib.oracle_mlp_adam(w.size, w, m, v, _f(g), lr, beta1, beta2, eps, c1, c2)
+    return w, m, v
+
+
+def bf16_round(x):
+    lib = load()
+    _mlp_protos(lib)
+    return np.float32(lib.oracle_bf16_round(float(x)))
