@@ -70,6 +70,16 @@ class gg_workspace(C.Structure):
     ]
 
 
+class gg_mlp_config(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("radius", "vscale", "guard_boundary", "guard_floor",
+                                         "guard_side")]
+
+
+class gg_mlp_weights(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("w_bf16", "bias", "w_master", "m_w", "v_w",
+                                          "b_master", "m_b", "v_b")]
+
+
 class gg_batch_scene(C.Structure):
     _fields_ = [("set", gg_particles), ("cell_start", C.c_void_p), ("boundary", gg_particles),
                 ("bcell_start", C.c_void_p), ("params", gg_params), ("ws", gg_workspace)]
@@ -122,6 +132,23 @@ _PROTOS = {
     "gg_batch_set_config": (C.c_int, [VP, P(gg_batch_config)]),
     "gg_batch_autotune": (C.c_int, [VP, C.c_int, VP, P(gg_batch_config), P(C.c_float)]),
     "gg_batch_destroy": (None, [VP]),
+    "gg_mlp_config_default": (None, [P(gg_mlp_config)]),
+    "gg_mlp_features": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, gg_grid,
+                                  P(gg_mlp_config), VP, VP, VP, VP]),
+    "gg_mlp_teacher": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, gg_grid,
+                                 P(gg_upsample_params), P(gg_mlp_config), VP, VP, VP, VP, VP]),
+    "gg_mlp_train_scratch_bytes": (C.c_size_t, [C.c_int]),
+    "gg_mlp_forward": (C.c_int, [P(gg_mlp_weights), VP, C.c_int, VP, VP, C.c_size_t, VP]),
+    "gg_mlp_grad": (C.c_int, [P(gg_mlp_weights), VP, VP, C.c_int, VP, VP, VP, VP, C.c_size_t,
+                              VP]),
+    "gg_mlp_adam": (C.c_int, [P(gg_mlp_weights), VP, VP, C.c_float, C.c_float, C.c_float,
+                              C.c_float, C.c_int, VP]),
+    "gg_mlp_sync_weights": (C.c_int, [P(gg_mlp_weights), VP]),
+    "gg_upsample_mlp_scratch_bytes": (C.c_size_t, [C.c_int]),
+    "gg_upsample_mlp": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, gg_grid,
+                                  P(gg_upsample_params), P(gg_mlp_config), VP, VP, VP, VP, VP,
+                                  C.c_size_t, P(C.c_int), VP]),
+    "gg_mfma_probe": (C.c_int, [VP, VP, VP, VP]),
     "gg_compact_scratch_bytes": (C.c_size_t, [C.c_longlong]),
     "gg_compact": (C.c_int, [VP, VP, C.c_longlong, C.c_int, VP, VP, VP, C.c_size_t, VP]),
     "gg_fill_u32": (C.c_int, [VP, C.c_size_t, C.c_uint32, VP]),

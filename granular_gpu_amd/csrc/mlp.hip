@@ -1,0 +1,680 @@
+// Learned upsampler on gfx950: kNN-16 features, the one-step teacher (the
+// plain upsampler applied to a 2x2x2 template), BF16 MFMA training kernels
+// with FP32 accumulation, and the fused inference kernel.
+//
+// Definitions shared with the oracle (features, flags, teacher, network,
+// mixed precision, loss) are stated in the public header.  No reference
+// code: the learned upsampler is an addition beside update_upsampled_cuda
+// (Solver.cu:1244-1387), whose update is the teacher.
+//
+// MFMA: v_mfma_f32_16x16x16_bf16 (one wave, 64 lanes).  Lane l = c + 16 q
+// (c = l % 16, q = l / 16) holds A[c][4q..4q+3], B[4q..4q+3][c] and
+// D[4q..4q+3][c].  The inference kernel computes each layer transposed,
+// Z^T = W H^T (weights as A, activations as B): the D tile of output
+// features 16t..16t+15 is then exactly the B operand of k-step t of the
+// next layer, so activations never leave registers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "gg_device.hpp"
+#include "gg_internal.hpp"
+#include "upsample_dev.hpp"
+
+namespace gg {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = 96, kHid = 128, kOut = 24, kOutPad = 32, kKnn = 16;
+constexpr int kWOff1 = 0, kWOff2 = 12288, kWOff3 = 28672, kWOff4 = 45056;
+constexpr int kBOff1 = 0, kBOff2 = 128, kBOff3 = 256, kBOff4 = 384;
+
+// Round to the nearest bfloat16 (ties to even); NaN stays NaN.
+__device__ __forceinline__ uint16_t to_bf16(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u | 0x00400000u) >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float templ(int k, int axis) {
+  return ((k >> (2 - axis)) & 1) ? 0.5f : -0.5f;
+}
+
+struct MlpConst {
+  float radius, vscale, guard_boundary, guard_floor, guard_side;
+};
+
+// ---- features ---------------------------------------------------------------
+// One thread per coarse particle: the 16 smallest (d2, j) of the stencil,
+// kept sorted in registers by an unrolled insertion.
+__global__ void __launch_bounds__(256)
+    k_mlp_features(int n, const float *__restrict__ pos, const float *__restrict__ vel,
+                   const int *__restrict__ cs, const float *__restrict__ bpos,
+                   const int *__restrict__ bcs, DevGrid g, MlpConst c,
+                   uint16_t *__restrict__ feat, int *__restrict__ flags, int *__restrict__ knn) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float xx = pos[3 * i], xy = pos[3 * i + 1], xz = pos[3 * i + 2];
+  float kd[kKnn];
+  int kj[kKnn];
+#pragma unroll
+  for (int s = 0; s < kKnn; ++s) {
+    kd[s] = __builtin_inff();
+    kj[s] = -1;
+  }
+  int flag = 0;
+  const float gb2 = c.guard_boundary * c.guard_boundary;
+  int bx, by, bz;
+  if (floor_cell(xx, xy, xz, g, bx, by, bz)) {
+    stencil_rows(bx, by, bz, g, cs, [&](int j0, int j1) {
+      for (int j = j0; j < j1; ++j) {
+        if (j == i) continue;
+        const float dx = pos[3 * j] - xx, dy = pos[3 * j + 1] - xy, dz = pos[3 * j + 2] - xz;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < kd[kKnn - 1] || (d2 == kd[kKnn - 1] && j < kj[kKnn - 1]))) continue;
+        // shift larger entries up by one, place (d2, j)
+        bool placed = false;
+#pragma unroll
+        for (int s = kKnn - 1; s > 0; --s) {
+          const bool larger = d2 < kd[s - 1] || (d2 == kd[s - 1] && j < kj[s - 1]);
+          if (!placed) {
+            if (larger) {
+              kd[s] = kd[s - 1];
+              kj[s] = kj[s - 1];
+            } else {
+              kd[s] = d2;
+              kj[s] = j;
+              placed = true;
+            }
+          }
+        }
+        if (!placed) {
+          kd[0] = d2;
+          kj[0] = j;
+        }
+      }
+    });
+    stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
+      for (int k = k0; k < k1; ++k) {
+        const float dx = bpos[3 * k] - xx, dy = bpos[3 * k + 1] - xy, dz = bpos[3 * k + 2] - xz;
+        if (dx * dx + dy * dy + dz * dz < gb2) flag |= 1;
+      }
+    });
+  }
+  if (xy < c.guard_floor || xx < c.guard_side || xz < c.guard_side) flag |= 2;
+  flags[i] = flag;
+  const float vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+  uint16_t *f = feat + (size_t)i * kIn;
+#pragma unroll
+  for (int s = 0; s < kKnn; ++s) {
+    const int j = kj[s];
+    if (knn) knn[(size_t)i * kKnn + s] = j;
+    uint16_t h[6] = {0, 0, 0, 0, 0, 0};
+    if (j >= 0) {
+      h[0] = to_bf16((pos[3 * j] - xx) / c.radius);
+      h[1] = to_bf16((pos[3 * j + 1] - xy) / c.radius);
+      h[2] = to_bf16((pos[3 * j + 2] - xz) / c.radius);
+      h[3] = to_bf16((vel[3 * j] - vx) / c.vscale);
+      h[4] = to_bf16((vel[3 * j + 1] - vy) / c.vscale);
+      h[5] = to_bf16((vel[3 * j + 2] - vz) / c.vscale);
+    }
+    // 6 halfwords = 3 words at an even halfword offset (96 per particle)
+    uint32_t *fw = reinterpret_cast<uint32_t *>(f + 6 * s);
+    fw[0] = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+    fw[1] = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+    fw[2] = (uint32_t)h[4] | ((uint32_t)h[5] << 16);
+  }
+}
+
+// ---- teacher ------------------------------------------------------------------
+// One thread per (coarse particle, template slot).  only_flagged: skip the
+// particles whose flag is 0 (the inference fallback).
+__global__ void __launch_bounds__(256)
+    k_mlp_teacher(int n, const float *__restrict__ pos, const float *__restrict__ vel,
+                  const int *__restrict__ cs, const float *__restrict__ bpos,
+                  const int *__restrict__ bcs, DevGrid g, UpConst u, float radius,
+                  const int *__restrict__ flags, float *__restrict__ y,
+                  float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8LL * n) return;
+  const int i = (int)(t >> 3), k = (int)(t & 7);
+  if (flags && flags[i] == 0) return;
+  const float vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+  const float px = pos[3 * i] + radius * templ(k, 0);
+  const float py = pos[3 * i + 1] + radius * templ(k, 1);
+  const float pz = pos[3 * i + 2] + radius * templ(k, 2);
+  float nx, ny, nz, wx, wy, wz;
+  upsample_one(px, py, pz, vx, vy, vz, pos, vel, cs, bpos, bcs, g, u, nx, ny, nz, wx, wy, wz);
+  const size_t o = (size_t)i * kOut + 3 * k;
+  if (y) {
+    y[o] = wx - vx;
+    y[o + 1] = wy - vy;
+    y[o + 2] = wz - vz;
+  }
+  if (fine_pos) {
+    fine_pos[o] = nx;
+    fine_pos[o + 1] = ny;
+    fine_pos[o + 2] = nz;
+  }
+  if (fine_vel) {
+    fine_vel[o] = wx;
+    fine_vel[o + 1] = wy;
+    fine_vel[o + 2] = wz;
+  }
+}
+
+// ---- training: generic bf16 GEMM (fp32 out) ---------------------------------
+// C[m][n] = sum_k A(m,k) B(k,n), A(m,k) = A[m sam + k sak],
+// B(k,n) = B[k sbk + n sbn]; one wave per 16x16 tile of C, 4 waves a block.
+// Bounds are checked per element (zero fill), so any M, N, K work.
+struct GemmArgs {
+  const uint16_t *A;
+  const uint16_t *B;
+  float *C;
+  int M, N, K;
+  long long sam, sak, sbk, sbn;
+  int ldc;
+};
+
+__global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs a) {
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  const int tiles_n = (a.N + 15) / 16;
+  const int tiles = ((a.M + 15) / 16) * tiles_n;
+  if (wave >= tiles) return;  // whole waves exit together
+  const int m0 = (wave / tiles_n) * 16, n0 = (wave % tiles_n) * 16;
+  const int c = lane & 15, q = lane >> 4;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < a.K; k0 += 16) {
+    bf16x4 av, bv;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = k0 + 4 * q + e;
+      const int m = m0 + c, nn = n0 + c;
+      av[e] = (m < a.M && k < a.K) ? (short)a.A[m * a.sam + k * a.sak] : (short)0;
+      bv[e] = (nn < a.N && k < a.K) ? (short)a.B[k * a.sbk + nn * a.sbn] : (short)0;
+    }
+    acc = mfma16(av, bv, acc);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int m = m0 + 4 * q + e, nn = n0 + c;
+    if (m < a.M && nn < a.N) a.C[(size_t)m * a.ldc + nn] = acc[e];
+  }
+}
+
+// Z = C + bias (fp32, kept for the backward mask); H = bf16(relu(Z)).
+// last: only Z (the network output y), N = 24 of the padded 32 columns.
+__global__ void k_bias_act(int B, int N, int ldc, const float *__restrict__ C,
+                           const float *__restrict__ bias, float *__restrict__ Z,
+                           uint16_t *__restrict__ H, int last) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)B * N) return;
+  const int b = (int)(t / N), nn = (int)(t % N);
+  const float z = C[(size_t)b * ldc + nn] + bias[nn];
+  Z[(size_t)b * N + nn] = z;
+  if (!last) H[(size_t)b * N + nn] = to_bf16(z > 0.0f ? z : 0.0f);
+}
+
+// g = 2 (y - t) / (24 B) on the 24 outputs, 0 on the 8 padding columns;
+// G = bf16(g); sq[b] = sum of the sample's squared errors.
+__global__ void k_loss_grad(int B, const float *__restrict__ y, const float *__restrict__ t,
+                            float *__restrict__ g, uint16_t *__restrict__ G,
+                            float *__restrict__ sq) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const float scale = 2.0f / (float)(kOut * B);
+  float s = 0.0f;
+  for (int nn = 0; nn < kOutPad; ++nn) {
+    float gv = 0.0f;
+    if (nn < kOut) {
+      const float e = y[(size_t)b * kOut + nn] - t[(size_t)b * kOut + nn];
+      s = s + e * e;
+      gv = e * scale;
+    }
+    g[(size_t)b * kOutPad + nn] = gv;
+    G[(size_t)b * kOutPad + nn] = to_bf16(gv);
+  }
+  if (sq) sq[b] = s;
+}
+
+// g_prev = D * [Z > 0]; G_prev = bf16(g_prev).
+__global__ void k_mask(long long count, const float *__restrict__ D, const float *__restrict__ Z,
+                       float *__restrict__ g, uint16_t *__restrict__ G) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const float v = Z[t] > 0.0f ? D[t] : 0.0f;
+  g[t] = v;
+  G[t] = to_bf16(v);
+}
+
+// db[n] = sum over b of g[b][n], in sample order (deterministic).
+__global__ void k_colsum(int B, int N, const float *__restrict__ g, float *__restrict__ db) {
+  const int nn = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nn >= N) return;
+  float s = 0.0f;
+  for (int b = 0; b < B; ++b) s = s + g[(size_t)b * N + nn];
+  db[nn] = s;
+}
+
+__global__ void k_adam(int count, float *__restrict__ w, float *__restrict__ m,
+                       float *__restrict__ v, const float *__restrict__ g,
+                       uint16_t *__restrict__ wb, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * (gi * gi);
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi * c1, vh = vi * c2;
+  const float wi = w[i] - (lr * mh) / (sqrtf(vh) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = to_bf16(wi);
+}
+
+__global__ void k_to_bf16(long long count, const float *__restrict__ x, uint16_t *__restrict__ h) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) h[t] = to_bf16(x[t]);
+}
+
+// ---- inference ----------------------------------------------------------------
+// Persistent blocks of 256 threads (4 waves); the weights (bf16, 96 KB) and
+// biases live in LDS.  A wave takes 32 samples per iteration (two 16-sample
+// MFMA column tiles that share every weight fragment).
+constexpr int kInferWaves = 4;
+constexpr int kSampTiles = 2;
+
+// One layer, transposed: output tile t (features 16t..16t+15) of both
+// sample tiles from KSTEPS input k-steps; bias, ReLU and bfloat16 packing
+// give the next layer's B operand directly.
+template <int KSTEPS>
+__device__ __forceinline__ void hidden_layer(const uint16_t *__restrict__ W,
+                                             const float *__restrict__ bias,
+                                             const bf16x4 (&in)[kSampTiles][8],
+                                             bf16x4 (&out)[kSampTiles][8], int lane) {
+  const int c = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    f32x4 a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+      const bf16x4 w = *reinterpret_cast<const bf16x4 *>(W + (16 * t + c) * (16 * KSTEPS) +
+                                                         16 * ks + 4 * q);
+      a0 = mfma16(w, in[0][ks], a0);
+      a1 = mfma16(w, in[1][ks], a1);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float b = bias[16 * t + 4 * q + e];
+      const float z0 = a0[e] + b, z1 = a1[e] + b;
+      out[0][t][e] = (short)to_bf16(z0 > 0.0f ? z0 : 0.0f);
+      out[1][t][e] = (short)to_bf16(z1 > 0.0f ? z1 : 0.0f);
+    }
+    // keep each tile's weight loads next to its MFMAs (bounds live registers)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Output layer: 2 tiles (24 outputs + 8 zero rows), no activation.
+__device__ __forceinline__ void output_layer(const uint16_t *__restrict__ W,
+                                             const float *__restrict__ bias,
+                                             const bf16x4 (&in)[kSampTiles][8],
+                                             f32x4 (&y)[kSampTiles][2], int lane) {
+  const int c = lane & 15, q = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    f32x4 a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const bf16x4 w = *reinterpret_cast<const bf16x4 *>(W + (16 * t + c) * 128 + 16 * ks + 4 * q);
+      a0 = mfma16(w, in[0][ks], a0);
+      a1 = mfma16(w, in[1][ks], a1);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float b = bias[16 * t + 4 * q + e];
+      y[0][t][e] = a0[e] + b;
+      y[1][t][e] = a1[e] + b;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_mlp_infer(int n, const uint16_t *__restrict__ feat, const int *__restrict__ flags,
+                const float *__restrict__ pos, const float *__restrict__ vel,
+                const uint16_t *__restrict__ wts, const float *__restrict__ bias_g, float radius,
+                float dt, float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
+  __shared__ uint16_t W[49152];
+  __shared__ float bias[416];
+  for (int i = threadIdx.x; i < 49152 / 2; i += blockDim.x)
+    reinterpret_cast<uint32_t *>(W)[i] = reinterpret_cast<const uint32_t *>(wts)[i];
+  for (int i = threadIdx.x; i < 416; i += blockDim.x) bias[i] = bias_g[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, q = lane >> 4;
+  const int per_iter = 16 * kSampTiles;
+  const int groups = (n + per_iter - 1) / per_iter;
+  for (int grp = blockIdx.x * kInferWaves + wave; grp < groups; grp += gridDim.x * kInferWaves) {
+    const int base = grp * per_iter;
+    bf16x4 h[kSampTiles][8], h2[kSampTiles][8];
+    f32x4 y[kSampTiles][2];
+    // layer 1 input: features k = 16 ks + 4 q + e of sample base + 16 s + c
+#pragma unroll
+    for (int s = 0; s < kSampTiles; ++s) {
+      const int smp = base + 16 * s + c;
+#pragma unroll
+      for (int ks = 0; ks < 6; ++ks) {
+        if (smp < n) {
+          h[s][ks] = *reinterpret_cast<const bf16x4 *>(feat + (size_t)smp * kIn + 16 * ks + 4 * q);
+        } else {
+          h[s][ks] = bf16x4{0, 0, 0, 0};
+        }
+      }
+      h[s][6] = bf16x4{0, 0, 0, 0};
+      h[s][7] = bf16x4{0, 0, 0, 0};
+    }
+    hidden_layer<6>(W + kWOff1, bias + kBOff1, h, h2, lane);
+    hidden_layer<8>(W + kWOff2, bias + kBOff2, h2, h, lane);
+    hidden_layer<8>(W + kWOff3, bias + kBOff3, h, h2, lane);
+    output_layer(W + kWOff4, bias + kBOff4, h2, y, lane);
+    // outputs f = 16 t + 4 q + e of sample base + 16 s + c
+#pragma unroll
+    for (int s = 0; s < kSampTiles; ++s) {
+      const int smp = base + 16 * s + c;
+      if (smp >= n || flags[smp] != 0) continue;
+      const float px = pos[3 * smp], py = pos[3 * smp + 1], pz = pos[3 * smp + 2];
+      const float vx = vel[3 * smp], vy = vel[3 * smp + 1], vz = vel[3 * smp + 2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int f = 16 * t + 4 * q + e;
+          if (f >= kOut) continue;
+          const int k = f / 3, ax = f % 3;
+          const float yv = y[s][t][e];
+          const float p0 = ax == 0 ? px : (ax == 1 ? py : pz);
+          const float v0 = ax == 0 ? vx : (ax == 1 ? vy : vz);
+          const float vf = v0 + yv;
+          const size_t o = (size_t)smp * kOut + f;
+          fine_pos[o] = (p0 + radius * templ(k, ax)) + dt * vf;
+          if (fine_vel) fine_vel[o] = vf;
+        }
+    }
+  }
+}
+
+// Layout probe: D = A B for one 16x16x16 tile (A, B row-major bf16).
+__global__ void k_mfma_probe(const uint16_t *__restrict__ A, const uint16_t *__restrict__ B,
+                             float *__restrict__ D) {
+  const int lane = threadIdx.x;
+  if (lane >= 64) return;
+  const int c = lane & 15, q = lane >> 4;
+  bf16x4 a, b;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    a[e] = (short)A[c * 16 + 4 * q + e];
+    b[e] = (short)B[(4 * q + e) * 16 + c];
+  }
+  f32x4 d = mfma16(a, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+  for (int e = 0; e < 4; ++e) D[(4 * q + e) * 16 + c] = d[e];
+}
+
+static inline int nblocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+MlpConst mlp_const(const gg_mlp_config *c) {
+  MlpConst m;
+  m.radius = c->radius;
+  m.vscale = c->vscale;
+  m.guard_boundary = c->guard_boundary;
+  m.guard_floor = c->guard_floor;
+  m.guard_side = c->guard_side;
+  return m;
+}
+
+int gemm(const uint16_t *A, long long sam, long long sak, const uint16_t *B, long long sbk,
+         long long sbn, float *C, int ldc, int M, int N, int K, hipStream_t st) {
+  if (M <= 0 || N <= 0) return GG_OK;
+  GemmArgs a{A, B, C, M, N, K, sam, sak, sbk, sbn, ldc};
+  const long long tiles = (long long)((M + 15) / 16) * ((N + 15) / 16);
+  k_gemm_bf16<<<nblocks(tiles, 4), 256, 0, st>>>(a);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+
+// Scratch of the training path for batch B.
+struct TrainScratch {
+  float *C, *Z[4], *g0, *g1, *dwork;
+  uint16_t *H[4], *G;
+};
+
+size_t train_scratch_layout(int B, TrainScratch *t, char *base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char *p = base ? base + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  const size_t b = (size_t)(B > 0 ? B : 0);
+  TrainScratch s{};
+  s.C = reinterpret_cast<float *>(take(b * kHid * 4));
+  for (int l = 0; l < 4; ++l) s.Z[l] = reinterpret_cast<float *>(take(b * kHid * 4));
+  s.g0 = reinterpret_cast<float *>(take(b * kHid * 4));
+  s.g1 = reinterpret_cast<float *>(take(b * kHid * 4));
+  s.dwork = reinterpret_cast<float *>(take(b * kHid * 4));
+  for (int l = 0; l < 4; ++l) s.H[l] = reinterpret_cast<uint16_t *>(take(b * kHid * 2));
+  s.G = reinterpret_cast<uint16_t *>(take(b * kHid * 2));
+  if (t) *t = s;
+  return off;
+}
+
+constexpr int kDimIn[4] = {kIn, kHid, kHid, kHid};
+constexpr int kDimOut[4] = {kHid, kHid, kHid, kOutPad};
+constexpr int kWOffs[4] = {kWOff1, kWOff2, kWOff3, kWOff4};
+constexpr int kBOffs[4] = {kBOff1, kBOff2, kBOff3, kBOff4};
+
+// Training-path forward: Z_l, H_l kept in scratch; y = Z_4 (B x 24).
+int train_forward(const uint16_t *wb, const float *bias, const uint16_t *x, int B,
+                  TrainScratch &s, hipStream_t st) {
+  const uint16_t *h = x;
+  int K = kIn;
+  for (int l = 0; l < 4; ++l) {
+    const int N = kDimOut[l];
+    // C[b][n] = sum_k h[b][k] W_l[n][k]
+    GG_TRY(gemm(h, K, 1, wb + kWOffs[l], 1, K, s.C, kHid, B, N, K, st));
+    const int Nz = (l == 3) ? kOut : N;
+    k_bias_act<<<nblocks((long long)B * Nz, 256), 256, 0, st>>>(B, Nz, kHid, s.C,
+                                                               bias + kBOffs[l], s.Z[l], s.H[l],
+                                                               l == 3 ? 1 : 0);
+    GG_LAUNCH_CHECK();
+    h = s.H[l];
+    K = N;
+  }
+  return GG_OK;
+}
+
+}  // namespace gg
+
+using namespace gg;
+
+extern "C" void gg_mlp_config_default(gg_mlp_config *c) {
+  if (!c) return;
+  c->radius = 0.02f;
+  c->vscale = 1.0f;
+  c->guard_boundary = 0.05f;
+  c->guard_floor = 0.08f;
+  c->guard_side = 0.05f;
+}
+
+extern "C" int gg_mlp_features(const gg_particles *coarse, const int *cs,
+                               const gg_particles *boundary, const int *bcs, gg_grid grid,
+                               const gg_mlp_config *c, uint16_t *feat, int *flags, int *knn,
+                               hipStream_t stream) {
+  if (!coarse || !cs || !boundary || !bcs || !c || !feat || !flags) return GG_ERR_INVALID;
+  if (coarse->n < 0 || grid.cx <= 0 || grid.cy <= 0 || grid.cz <= 0) return GG_ERR_INVALID;
+  if (!(c->radius > 0.0f) || !(c->vscale > 0.0f)) return GG_ERR_INVALID;
+  if (coarse->n == 0) return GG_OK;
+  if (!coarse->pos || !coarse->vel || (boundary->n > 0 && !boundary->pos)) return GG_ERR_INVALID;
+  k_mlp_features<<<nblocks(coarse->n, 256), 256, 0, stream>>>(
+      coarse->n, reinterpret_cast<const float *>(coarse->pos),
+      reinterpret_cast<const float *>(coarse->vel), cs,
+      reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), mlp_const(c), feat,
+      flags, knn);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+
+extern "C" int gg_mlp_teacher(const gg_particles *coarse, const int *cs,
+                              const gg_particles *boundary, const int *bcs, gg_grid grid,
+                              const gg_upsample_params *u, const gg_mlp_config *c,
+                              const int *flags, float *y, float *fine_pos, float *fine_vel,
+                              hipStream_t stream) {
+  if (!coarse || !cs || !boundary || !bcs || !u || !c) return GG_ERR_INVALID;
+  if (coarse->n < 0 || grid.cx <= 0 || grid.cy <= 0 || grid.cz <= 0) return GG_ERR_INVALID;
+  if (coarse->n == 0) return GG_OK;
+  if (!coarse->pos || !coarse->vel || (boundary->n > 0 && !boundary->pos)) return GG_ERR_INVALID;
+  k_mlp_teacher<<<nblocks(8LL * coarse->n, 256), 256, 0, stream>>>(
+      coarse->n, reinterpret_cast<const float *>(coarse->pos),
+      reinterpret_cast<const float *>(coarse->vel), cs,
+      reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), up_const(u),
+      c->radius, flags, y, fine_pos, fine_vel);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+
+extern "C" size_t gg_mlp_train_scratch_bytes(int batch) {
+  if (batch <= 0) return 0;
+  return train_scratch_layout(batch, nullptr, nullptr);
+}
+
+extern "C" int gg_mlp_forward(const gg_mlp_weights *w, const uint16_t *x, int B, float *y,
+                              void *scratch, size_t scratch_bytes, hipStream_t stream) {
+  if (!w || !w->w_bf16 || !w->bias || !x || !y || B <= 0 || !scratch) return GG_ERR_INVALID;
+  if (scratch_bytes < gg_mlp_train_scratch_bytes(B)) return GG_ERR_WORKSPACE;
+  TrainScratch s;
+  train_scratch_layout(B, &s, static_cast<char *>(scratch));
+  GG_TRY(train_forward(w->w_bf16, w->bias, x, B, s, stream));
+  return hip_status(hipMemcpyAsync(y, s.Z[3], sizeof(float) * (size_t)B * kOut,
+                                   hipMemcpyDeviceToDevice, stream));
+}
+
+extern "C" int gg_mlp_grad(const gg_mlp_weights *w, const uint16_t *x, const float *target,
+                           int B, float *dW, float *db, float *sq_err, void *scratch,
+                           size_t scratch_bytes, hipStream_t stream) {
+  if (!w || !w->w_bf16 || !w->bias || !x || !target || !dW || !db || B <= 0 || !scratch)
+    return GG_ERR_INVALID;
+  if (scratch_bytes < gg_mlp_train_scratch_bytes(B)) return GG_ERR_WORKSPACE;
+  TrainScratch s;
+  train_scratch_layout(B, &s, static_cast<char *>(scratch));
+  GG_TRY(train_forward(w->w_bf16, w->bias, x, B, s, stream));
+  // output layer: g (B x 32), G = bf16(g)
+  k_loss_grad<<<nblocks(B, 256), 256, 0, stream>>>(B, s.Z[3], target, s.g0, s.G, sq_err);
+  GG_LAUNCH_CHECK();
+  float *g = s.g0, *gp = s.g1;
+  for (int l = 3; l >= 0; --l) {
+    const int K = kDimIn[l], N = kDimOut[l];
+    const uint16_t *hin = (l == 0) ? x : s.H[l - 1];
+    // dW_l[n][k] = sum_b G[b][n] h[b][k]
+    GG_TRY(gemm(s.G, 1, N, hin, K, 1, dW + kWOffs[l], K, N, K, B, stream));
+    k_colsum<<<nblocks(N, 128), 128, 0, stream>>>(B, N, g, db + kBOffs[l]);
+    GG_LAUNCH_CHECK();
+    if (l == 0) break;
+    // D[b][k] = sum_n G[b][n] W_l[n][k]; g_prev = D * [Z_{l-1} > 0]
+    GG_TRY(gemm(s.G, N, 1, w->w_bf16 + kWOffs[l], K, 1, s.dwork, K, B, K, N, stream));
+    k_mask<<<nblocks((long long)B * K, 256), 256, 0, stream>>>((long long)B * K, s.dwork,
+                                                               s.Z[l - 1], gp, s.G);
+    GG_LAUNCH_CHECK();
+    float *tmp = g;
+    g = gp;
+    gp = tmp;
+  }
+  return GG_OK;
+}
+
+extern "C" int gg_mlp_adam(gg_mlp_weights *w, const float *dW, const float *db, float lr,
+                           float beta1, float beta2, float eps, int step, hipStream_t stream) {
+  if (!w || !w->w_master || !w->m_w || !w->v_w || !w->b_master || !w->m_b || !w->v_b ||
+      !w->w_bf16 || !w->bias || !dW || !db || step < 1)
+    return GG_ERR_INVALID;
+  const float c1 = (float)(1.0 / (1.0 - std::pow((double)beta1, step)));
+  const float c2 = (float)(1.0 / (1.0 - std::pow((double)beta2, step)));
+  k_adam<<<nblocks(GG_MLP_WEIGHTS, 256), 256, 0, stream>>>(GG_MLP_WEIGHTS, w->w_master, w->m_w,
+                                                          w->v_w, dW, w->w_bf16, lr, beta1,
+                                                          beta2, eps, c1, c2);
+  GG_LAUNCH_CHECK();
+  k_adam<<<nblocks(GG_MLP_BIASES, 256), 256, 0, stream>>>(GG_MLP_BIASES, w->b_master, w->m_b,
+                                                         w->v_b, db, nullptr, lr, beta1, beta2,
+                                                         eps, c1, c2);
+  GG_LAUNCH_CHECK();
+  return hip_status(hipMemcpyAsync(w->bias, w->b_master, sizeof(float) * GG_MLP_BIASES,
+                                   hipMemcpyDeviceToDevice, stream));
+}
+
+extern "C" int gg_mlp_sync_weights(gg_mlp_weights *w, hipStream_t stream) {
+  if (!w || !w->w_master || !w->w_bf16 || !w->b_master || !w->bias) return GG_ERR_INVALID;
+  k_to_bf16<<<nblocks(GG_MLP_WEIGHTS, 256), 256, 0, stream>>>(GG_MLP_WEIGHTS, w->w_master,
+                                                             w->w_bf16);
+  GG_LAUNCH_CHECK();
+  return hip_status(hipMemcpyAsync(w->bias, w->b_master, sizeof(float) * GG_MLP_BIASES,
+                                   hipMemcpyDeviceToDevice, stream));
+}
+
+extern "C" size_t gg_upsample_mlp_scratch_bytes(int n) {
+  if (n < 0) return 0;
+  const size_t feat = ((size_t)n * kIn * 2 + 255) & ~(size_t)255;
+  const size_t fl = ((size_t)n * 4 + 255) & ~(size_t)255;
+  return feat + fl + 256;
+}
+
+extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
+                               const gg_particles *boundary, const int *bcs, gg_grid grid,
+                               const gg_upsample_params *u, const gg_mlp_config *c,
+                               const uint16_t *w_bf16, const float *bias, float *fine_pos,
+                               float *fine_vel, void *scratch, size_t scratch_bytes,
+                               int *fallback_count, hipStream_t stream) {
+  if (!coarse || !cs || !boundary || !bcs || !u || !c || !w_bf16 || !bias || !fine_pos ||
+      !scratch)
+    return GG_ERR_INVALID;
+  const int n = coarse->n;
+  if (n < 0) return GG_ERR_INVALID;
+  if (scratch_bytes < gg_upsample_mlp_scratch_bytes(n)) return GG_ERR_WORKSPACE;
+  if (n == 0) return GG_OK;
+  uint16_t *feat = static_cast<uint16_t *>(scratch);
+  int *flags = reinterpret_cast<int *>(static_cast<char *>(scratch) +
+                                       (((size_t)n * kIn * 2 + 255) & ~(size_t)255));
+  GG_TRY(gg_mlp_features(coarse, cs, boundary, bcs, grid, c, feat, flags, nullptr, stream));
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int groups = (n + 16 * kSampTiles - 1) / (16 * kSampTiles);
+  int grid_blocks = (groups + kInferWaves - 1) / kInferWaves;
+  if (grid_blocks > cus) grid_blocks = cus;
+  k_mlp_infer<<<grid_blocks, 256, 0, stream>>>(n, feat, flags,
+                                               reinterpret_cast<const float *>(coarse->pos),
+                                               reinterpret_cast<const float *>(coarse->vel),
+                                               w_bf16, bias, c->radius, u->dt, fine_pos,
+                                               fine_vel);
+  GG_LAUNCH_CHECK();
+  // flagged particles: the teacher (the plain upsampler on the template)
+  GG_TRY(gg_mlp_teacher(coarse, cs, boundary, bcs, grid, u, c, flags, nullptr, fine_pos,
+                        fine_vel, stream));
+  (void)fallback_count;
+  return GG_OK;
+}
+
+extern "C" int gg_mfma_probe(const uint16_t *a, const uint16_t *b, float *d, hipStream_t stream) {
+  if (!a || !b || !d) return GG_ERR_INVALID;
+  k_mfma_probe<<<1, 64, 0, stream>>>(a, b, d);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}

@@ -14,27 +14,9 @@
 #include "gg_internal.hpp"
 #include "grid_kernels.hpp"
 #include "profiler.hpp"
+#include "upsample_dev.hpp"
 
 namespace gg {
-
-struct UpConst {
-  float dt, gdx, gdy, gdz, r9, brad, rep, minw, neard, groundy, groundd, vmax, floory, floorf;
-};
-
-__device__ __forceinline__ void coarse_term(float px, float py, float pz, float cx, float cy,
-                                            float cz, float vx, float vy, float vz, float r9,
-                                            float &gw, float &wx, float &wy, float &wz,
-                                            float &maxw) {
-  const float dx = px - cx, dy = py - cy, dz = pz - cz;
-  const float dis = sqrtf(dx * dx + dy * dy + dz * dz);
-  const float t1 = 1.0f - dis * dis * r9;
-  const float w = fmaxf(0.0f, t1 * t1 * t1);
-  gw = gw + w;
-  wx = wx + w * vx;
-  wy = wy + w * vy;
-  wz = wz + w * vz;
-  maxw = fmaxf(maxw, w);
-}
 
 __global__ void __launch_bounds__(kBlock)
     k_upsample(int nf, float *__restrict__ fpos, float *__restrict__ fvel,
@@ -43,90 +25,9 @@ __global__ void __launch_bounds__(kBlock)
                const int *__restrict__ bcs, DevGrid g, UpConst u, int remap) {
   const int i = xcd_block(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
   if (i >= nf) return;
-  const float px = fpos[3 * i], py = fpos[3 * i + 1], pz = fpos[3 * i + 2];
-  float wx = 0.0f, wy = 0.0f, wz = 0.0f, gw = 0.0f, maxw = 0.0f;
-  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-  bool near = false;
-  int bx, by, bz;
-  if (floor_cell(px, py, pz, g, bx, by, bz)) {
-    stencil_rows(bx, by, bz, g, cs, [&](int j0, int j1) {
-      int j = j0;
-      for (; j + 2 <= j1; j += 2) {
-        const float a0 = cpos[3 * j], a1 = cpos[3 * j + 1], a2 = cpos[3 * j + 2];
-        const float b0 = cpos[3 * j + 3], b1 = cpos[3 * j + 4], b2 = cpos[3 * j + 5];
-        const float v0 = cvel[3 * j], v1 = cvel[3 * j + 1], v2 = cvel[3 * j + 2];
-        const float w0 = cvel[3 * j + 3], w1 = cvel[3 * j + 4], w2 = cvel[3 * j + 5];
-        coarse_term(px, py, pz, a0, a1, a2, v0, v1, v2, u.r9, gw, wx, wy, wz, maxw);
-        coarse_term(px, py, pz, b0, b1, b2, w0, w1, w2, u.r9, gw, wx, wy, wz, maxw);
-      }
-      for (; j < j1; ++j)
-        coarse_term(px, py, pz, cpos[3 * j], cpos[3 * j + 1], cpos[3 * j + 2], cvel[3 * j],
-                    cvel[3 * j + 1], cvel[3 * j + 2], u.r9, gw, wx, wy, wz, maxw);
-    });
-    stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
-      for (int k = k0; k < k1; ++k) {
-        const float tx = px - bpos[3 * k], ty = py - bpos[3 * k + 1], tz = pz - bpos[3 * k + 2];
-        const float dis = sqrtf(tx * tx + ty * ty + tz * tz);
-        if (dis < u.brad) {
-          near = true;
-          const float den = dis + 1e-6f;
-          const float force = (u.rep * (u.brad - dis)) / u.brad;
-          rx = rx + (tx / den) * force;
-          ry = ry + (ty / den) * force;
-          rz = rz + (tz / den) * force;
-        }
-      }
-    });
-  }
-  const float vix = fvel[3 * i], viy = fvel[3 * i + 1], viz = fvel[3 * i + 2];
-  float vx, vy, vz;
-  if (gw > u.minw) {
-    wx = wx / gw;
-    wy = wy / gw;
-    wz = wz / gw;
-    const float a = fmaxf(0.0f, 1.0f - maxw);
-    vx = a * (vix + u.gdx) + (1.0f - a) * wx;
-    vy = a * (viy + u.gdy) + (1.0f - a) * wy;
-    vz = a * (viz + u.gdz) + (1.0f - a) * wz;
-  } else {
-    vx = vix + u.gdx;
-    vy = viy + u.gdy;
-    vz = viz + u.gdz;
-    if (near) {
-      vx = vx * u.neard;
-      vy = vy * u.neard;
-      vz = vz * u.neard;
-      if (py < u.groundy) vy = vy * u.groundd;
-    }
-  }
-  vx = vx + rx;
-  vy = vy + ry;
-  vz = vz + rz;
-  vx = fmaxf(-u.vmax, fminf(vx, u.vmax));
-  vy = fmaxf(-u.vmax, fminf(vy, u.vmax));
-  vz = fmaxf(-u.vmax, fminf(vz, u.vmax));
-  // v_min/v_max return the non-NaN operand: a NaN leaves the clamp as
-  // +vmax, so (as in the reference) this test never fires
-  if (isnan(vx) || isnan(vy) || isnan(vz)) {
-    vx = 0.0f;
-    vy = 0.0f;
-    vz = 0.0f;
-  }
-  float nx = px + vx * u.dt, ny = py + vy * u.dt, nz = pz + vz * u.dt;
-  if (ny < u.floory) {
-    ny = u.floory;
-    vy = 0.0f;
-    vx = vx * u.floorf;
-    vz = vz * u.floorf;
-  }
-  if (nx < 0.0f) {
-    nx = 0.0f;
-    vx = 0.0f;
-  }
-  if (nz < 0.0f) {
-    nz = 0.0f;
-    vz = 0.0f;
-  }
+  float nx, ny, nz, vx, vy, vz;
+  upsample_one(fpos[3 * i], fpos[3 * i + 1], fpos[3 * i + 2], fvel[3 * i], fvel[3 * i + 1],
+               fvel[3 * i + 2], cpos, cvel, cs, bpos, bcs, g, u, nx, ny, nz, vx, vy, vz);
   fpos[3 * i] = nx;
   fpos[3 * i + 1] = ny;
   fpos[3 * i + 2] = nz;
@@ -203,21 +104,7 @@ extern "C" int gg_upsample_plain(gg_particles *fine, const gg_particles *coarse,
   if (!fine->pos || !fine->vel || (coarse->n > 0 && (!coarse->pos || !coarse->vel)) ||
       (boundary->n > 0 && !boundary->pos))
     return GG_ERR_INVALID;
-  UpConst u;
-  u.dt = up->dt;
-  u.gdx = up->gx * up->dt;
-  u.gdy = up->gy * up->dt;
-  u.gdz = up->gz * up->dt;
-  u.r9 = up->r9;
-  u.brad = up->boundary_radius;
-  u.rep = up->repulsion;
-  u.minw = up->min_weight;
-  u.neard = up->near_damping;
-  u.groundy = up->ground_y;
-  u.groundd = up->ground_damping;
-  u.vmax = up->max_velocity;
-  u.floory = up->floor_y;
-  u.floorf = up->floor_friction;
+  const UpConst u = up_const(up);
   prof_begin(GG_STAGE_UPSAMPLE, stream);
   k_upsample<<<blocks_for(fine->n), kBlock, 0, stream>>>(
       fine->n, reinterpret_cast<float *>(fine->pos), reinterpret_cast<float *>(fine->vel),

@@ -282,6 +282,81 @@ int gg_profiler_collect(double *total_ms, int *count, int nstages);
  * particles of sweep k that scanned the full stencil. */
 const void *gg_workspace_view(const gg_workspace *ws, int which);
 
+/* ---- learned upsampler ------------------------------------------------------ */
+/* No reference counterpart (a specification addition beside
+ * update_upsampled_cuda, Solver.cu:1244-1387).  Definitions:
+ * - features of coarse particle i: its 16 nearest coarse particles j != i
+ *   in the 27-cell stencil of floor(x_i/L), ordered by (|x_j - x_i|^2, j);
+ *   slot s = ((x_j - x_i)/radius, (v_j - v_i)/vscale), zeros when fewer
+ *   than 16; 96 bfloat16 values;
+ * - flags: bit 0 = a boundary particle closer than guard_boundary; bit 1 =
+ *   y < guard_floor or x, z < guard_side.  Flagged particles take the
+ *   teacher's result (the fallback);
+ * - teacher: the plain upsampler's update applied to the 8 template points
+ *   x_i + radius (+-1/2, +-1/2, +-1/2) (slot k = 4 sx + 2 sy + sz) with the
+ *   parent's velocity; target y_k = v_fine - v_i;
+ * - network 96 -> 128 -> 128 -> 128 -> 24, ReLU; bfloat16 weights and layer
+ *   inputs, FP32 accumulation and biases; weights [n_out][k_in] per layer at
+ *   offsets 0, 12288, 28672, 45056 (layer 4 padded to 32 rows), biases at
+ *   0, 128, 256, 384;
+ * - fine position of slot k: (x_i + radius T_k) + dt (v_i + y_k); fine
+ *   velocity v_i + y_k (dt from gg_upsample_params). */
+#define GG_MLP_WEIGHTS 49152
+#define GG_MLP_BIASES 416
+typedef struct gg_mlp_config {
+  float radius;          /* coarse radius (0.02)                 */
+  float vscale;          /* velocity scale, m/s (1.0)            */
+  float guard_boundary;  /* 0.05                                  */
+  float guard_floor;     /* 0.08                                  */
+  float guard_side;      /* 0.05                                  */
+} gg_mlp_config;
+typedef struct gg_mlp_weights {
+  uint16_t *w_bf16;  /* GG_MLP_WEIGHTS bfloat16 (inference and training)  */
+  float *bias;       /* GG_MLP_BIASES                                      */
+  float *w_master;   /* FP32 master weights (training)                     */
+  float *m_w, *v_w;  /* Adam moments of the weights                        */
+  float *b_master;   /* FP32 master biases                                 */
+  float *m_b, *v_b;  /* Adam moments of the biases                         */
+} gg_mlp_weights;
+void gg_mlp_config_default(gg_mlp_config *c);
+/* feat: n x 96 bfloat16; flags: n; knn (optional): n x 16 indices, -1 pad. */
+int gg_mlp_features(const gg_particles *coarse, const int *cell_start,
+                    const gg_particles *boundary, const int *bcell_start, gg_grid grid,
+                    const gg_mlp_config *c, uint16_t *feat, int *flags, int *knn,
+                    hipStream_t stream);
+/* Teacher for every particle (flags NULL) or only the flagged ones; each
+ * output (n x 24) optional. */
+int gg_mlp_teacher(const gg_particles *coarse, const int *cell_start,
+                   const gg_particles *boundary, const int *bcell_start, gg_grid grid,
+                   const gg_upsample_params *u, const gg_mlp_config *c, const int *flags,
+                   float *y, float *fine_pos, float *fine_vel, hipStream_t stream);
+/* Training path (MFMA GEMMs, scratch of gg_mlp_train_scratch_bytes(B)). */
+size_t gg_mlp_train_scratch_bytes(int batch);
+int gg_mlp_forward(const gg_mlp_weights *w, const uint16_t *x, int batch, float *y,
+                   void *scratch, size_t scratch_bytes, hipStream_t stream);
+/* Loss gradients of mean((y - t)^2) over batch x 24: dW (GG_MLP_WEIGHTS),
+ * db (GG_MLP_BIASES), per-sample squared error sum sq_err (optional). */
+int gg_mlp_grad(const gg_mlp_weights *w, const uint16_t *x, const float *target, int batch,
+                float *dW, float *db, float *sq_err, void *scratch, size_t scratch_bytes,
+                hipStream_t stream);
+/* Adam on the FP32 masters (bias-corrected, step >= 1); refreshes the
+ * bfloat16 weights and the biases. */
+int gg_mlp_adam(gg_mlp_weights *w, const float *dW, const float *db, float lr, float beta1,
+                float beta2, float eps, int step, hipStream_t stream);
+/* bfloat16 weights and biases from the FP32 masters. */
+int gg_mlp_sync_weights(gg_mlp_weights *w, hipStream_t stream);
+/* Inference: features, fused register-resident MFMA network, teacher on
+ * the flagged particles.  fine_pos / fine_vel: n x 24 (8 fine per coarse,
+ * fine_vel optional); scratch: gg_upsample_mlp_scratch_bytes(n). */
+size_t gg_upsample_mlp_scratch_bytes(int n);
+int gg_upsample_mlp(const gg_particles *coarse, const int *cell_start,
+                    const gg_particles *boundary, const int *bcell_start, gg_grid grid,
+                    const gg_upsample_params *u, const gg_mlp_config *c, const uint16_t *w_bf16,
+                    const float *bias, float *fine_pos, float *fine_vel, void *scratch,
+                    size_t scratch_bytes, int *fallback_count, hipStream_t stream);
+/* One 16x16x16 bfloat16 MFMA, D = A B (row-major): checks the lane layout. */
+int gg_mfma_probe(const uint16_t *a, const uint16_t *b, float *d, hipStream_t stream);
+
 /* ---- batched scenes -------------------------------------------------------- */
 /* Many independent plain scenes advanced together by one hipGraph launch
  * (no reference counterpart: the specification's batched-scenes

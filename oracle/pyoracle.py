@@ -350,8 +350,11 @@ def mlp_backward(x, W, b, t, exact=0):
 def mlp_adam(w, m, v, g, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
     lib = load()
     _mlp_protos(lib)
-    c1 = float(np.float32(1.0 / (1.0 - beta1 ** step)))
-    c2 = float(np.float32(1.0 / (1.0 - beta2 ** step)))
+    # bias corrections as the device library forms them: in double from the
+    # float32 betas, rounded once
+    b1d, b2d = float(np.float32(beta1)), float(np.float32(beta2))
+    c1 = float(np.float32(1.0 / (1.0 - b1d ** step)))
+    c2 = float(np.float32(1.0 / (1.0 - b2d ** step)))
     w, m, v = (np.ascontiguousarray(a, np.float32).copy() for a in (w, m, v))
     lib.oracle_mlp_adam(w.size, w, m, v, _f(g), lr, beta1, beta2, eps, c1, c2)
     return w, m, v
