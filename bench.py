@@ -44,7 +44,14 @@ def upsample_phase(args, lib, torch, C, np):
     upsampler pass; fine-particles/s counts the fine work (sort + pass) from
     the stage timers.  Upsampler algorithmic bytes per launch:
     48 N_f (read+write fine pos, vel) + 24 N (coarse pos, vel) + 12 N_b +
-    8 C (two cell tables)."""
+    8 C (two cell tables).
+
+    The learned upsampler runs on the same states: trained on the GPU on 50k
+    pairs from 3 snapshots after the warm-up, then timed beside the plain
+    one (features + network + teacher fallback on flagged particles) and
+    judged against the teacher at the end.  Its network roofline counts
+    2 (96 128 + 2 128 128 + 128 24) = 96,256 flops per coarse particle
+    against the 2.5 PFLOP/s dense bf16 peak."""
     from granular_gpu_amd import _lib, scenes
     from granular_gpu_amd.device import FineSet, PlainSim
 
@@ -59,17 +66,40 @@ def upsample_phase(args, lib, torch, C, np):
     print(f"# upsample scene: coarse={n} fine={nf} walls={nb} cells={cells}", file=sys.stderr,
           flush=True)
 
-    def run(k):
+    learned = None
+    if not args.no_learned:
+        from granular_gpu_amd.learned import LearnedUpsampler, MlpNet, Trainer
+        learned = LearnedUpsampler(MlpNet(seed=6))
+
+    def run(k, with_learned=False):
         for _ in range(k):
             sim.step()
             fs.sort()
             fs.upsample()
+            if with_learned:
+                learned.infer(sim, fine_pos=lpos)
 
     run(args.upsample_warmup)
+    train = None
+    if learned is not None:
+        # training pairs: 3 snapshots 10 steps apart, 50k pairs drawn from them
+        xs, ts = learned.collect(sim, snapshots=3, every=10)
+        g = torch.Generator(device="cpu").manual_seed(1)
+        pick = torch.randperm(xs.shape[0], generator=g)[:50000].to(xs.device)
+        xs, ts = xs[pick].contiguous(), ts[pick].contiguous()
+        tr = Trainer(learned.net, batch=1024)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        losses = tr.fit(xs, ts, steps=args.learned_train_steps, lr=2e-3, lr_final=1e-4, seed=0)
+        torch.cuda.synchronize()
+        train = {"pairs": int(xs.shape[0]), "steps": args.learned_train_steps, "batch": 1024,
+                 "seconds": round(time.perf_counter() - t0, 3),
+                 "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-50:].mean())}
+        lpos = torch.empty((sim.n, 24), dtype=torch.float32, device=sim.pos.device)
     torch.cuda.synchronize()
-    lib.gg_profiler_enable(args.upsample_steps * 24 + 16)
+    lib.gg_profiler_enable(args.upsample_steps * 32 + 16)
     t0 = time.perf_counter()
-    run(args.upsample_steps)
+    run(args.upsample_steps, with_learned=learned is not None)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     tot = (C.c_double * len(_lib.STAGES))()
@@ -88,6 +118,37 @@ def upsample_phase(args, lib, torch, C, np):
     ach = per_launch / (up_us * 1e-6) / 1e9
     print(f"# upsample timed {args.upsample_steps} steps in {el:.3f} s", file=sys.stderr,
           flush=True)
+    learned_out = None
+    if learned is not None:
+        from granular_gpu_amd.learned import fine_error
+        _, tfp, _ = learned.teacher(sim)
+        r_fine = 0.01
+        e_l = fine_error(lpos, tfp) / r_fine
+        e_r = fine_error(learned.rigid(sim), tfp) / r_fine
+        fl = learned.features(sim)[1]
+        nflag = int((fl != 0).sum().item())
+        t_net = st["mlp"]["avg_us"]
+        t_feat = st["mlp_features"]["avg_us"]
+        t_fb = st["mlp_fallback"]["avg_us"]
+        t_all = t_net + t_feat + t_fb
+        flops = 96256 * n
+        ach = flops / (t_net * 1e-6) / 1e12
+        learned_out = {
+            "network": "96-128-128-128-24 ReLU, bf16 weights/activations, fp32 accumulate",
+            "training": train,
+            "fine_particles_per_s": round(nf / (t_all * 1e-6), 1),
+            "speedup_vs_plain": round((up_us + sort_us) / t_all, 3),
+            "stage_us": {"features": t_feat, "network": t_net, "fallback": t_fb},
+            "fallback_coarse": nflag,
+            "error_vs_teacher_r_fine": round(e_l, 5),
+            "rigid_template_error_r_fine": round(e_r, 5),
+            "roofline": {"kernel": "k_mlp_infer (fused MFMA network)", "bound": "mfma",
+                         "achieved": round(ach, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(ach / 2500.0, 4),
+                         "algorithmic_flops_per_launch": flops, "avg_launch_us": t_net},
+        }
+        print(f"# learned: {1e-9 * nf / (t_all * 1e-6):.2f} G fine/s, error {e_l:.4f} r_fine "
+              f"(rigid {e_r:.4f})", file=sys.stderr, flush=True)
     return {
         "workload": f"{side}^3 coarse lattice r=0.02 (m=8) in a {box}^3 m box, {per} fine "
                     f"per coarse (offsets U(+-0.02), seed 7), {scene.cells[0]}^3 cells, "
@@ -102,6 +163,7 @@ def upsample_phase(args, lib, torch, C, np):
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4),
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": up_us},
+        "learned": learned_out,
     }
 
 
@@ -172,6 +234,8 @@ def main():
     ap.add_argument("--upsample-side", type=int, default=63, help="63 -> 250,047 coarse")
     ap.add_argument("--upsample-steps", type=int, default=100)
     ap.add_argument("--upsample-warmup", type=int, default=20)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-train-steps", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)

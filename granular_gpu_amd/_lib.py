@@ -21,7 +21,7 @@ GG_ERR_WORKSPACE = -4
 GG_ERR_NO_DEVICE = -5
 GG_FLAG_NO_NLIST = 1
 STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep", "adaptive", "upsample",
-          "upsample_sort", "mlp"]
+          "upsample_sort", "mlp", "mlp_features", "mlp_fallback"]
 
 
 class gg_grid(C.Structure):

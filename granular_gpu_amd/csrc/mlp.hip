@@ -20,6 +20,7 @@
 
 #include "gg_device.hpp"
 #include "gg_internal.hpp"
+#include "profiler.hpp"
 #include "upsample_dev.hpp"
 
 namespace gg {
@@ -652,22 +653,28 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   uint16_t *feat = static_cast<uint16_t *>(scratch);
   int *flags = reinterpret_cast<int *>(static_cast<char *>(scratch) +
                                        (((size_t)n * kIn * 2 + 255) & ~(size_t)255));
+  prof_begin(GG_STAGE_MLP_FEATURES, stream);
   GG_TRY(gg_mlp_features(coarse, cs, boundary, bcs, grid, c, feat, flags, nullptr, stream));
+  prof_end(GG_STAGE_MLP_FEATURES, stream);
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const int groups = (n + 16 * kSampTiles - 1) / (16 * kSampTiles);
   int grid_blocks = (groups + kInferWaves - 1) / kInferWaves;
   if (grid_blocks > cus) grid_blocks = cus;
+  prof_begin(GG_STAGE_MLP, stream);
   k_mlp_infer<<<grid_blocks, 256, 0, stream>>>(n, feat, flags,
                                                reinterpret_cast<const float *>(coarse->pos),
                                                reinterpret_cast<const float *>(coarse->vel),
                                                w_bf16, bias, c->radius, u->dt, fine_pos,
                                                fine_vel);
   GG_LAUNCH_CHECK();
+  prof_end(GG_STAGE_MLP, stream);
   // flagged particles: the teacher (the plain upsampler on the template)
+  prof_begin(GG_STAGE_MLP_FALLBACK, stream);
   GG_TRY(gg_mlp_teacher(coarse, cs, boundary, bcs, grid, u, c, flags, nullptr, fine_pos,
                         fine_vel, stream));
+  prof_end(GG_STAGE_MLP_FALLBACK, stream);
   (void)fallback_count;
   return GG_OK;
 }

@@ -267,8 +267,10 @@ enum {
   GG_STAGE_ADAPTIVE = 6, /* surface / distance / merge / split passes */
   GG_STAGE_UPSAMPLE = 7, /* plain upsampler pass */
   GG_STAGE_UPSAMPLE_SORT = 8, /* fine-particle cell sort */
-  GG_STAGE_MLP = 9,      /* learned upsampler (features + MLP) */
-  GG_STAGE_COUNT = 10
+  GG_STAGE_MLP = 9,      /* learned upsampler: fused MFMA network */
+  GG_STAGE_MLP_FEATURES = 10, /* learned upsampler: kNN-16 features  */
+  GG_STAGE_MLP_FALLBACK = 11, /* learned upsampler: teacher fallback */
+  GG_STAGE_COUNT = 12
 };
 int gg_profiler_enable(int max_records); /* 0 disables */
 int gg_profiler_collect(double *total_ms, int *count, int nstages);
