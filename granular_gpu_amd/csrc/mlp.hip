@@ -56,11 +56,17 @@ struct MlpConst {
 // ---- features ---------------------------------------------------------------
 // One thread per coarse particle: the 16 smallest (d2, j) of the stencil,
 // kept sorted in registers by an unrolled insertion.
+// GROUPED = 0: feat[i][96] (sample-major, the training layout);
+// GROUPED = 1: feat[g][i][4] for the 24 groups g of 4 consecutive features
+// (the inference layout: a wave's stores and the network's B-operand loads
+// are contiguous).
+template <int GROUPED>
 __global__ void __launch_bounds__(256)
     k_mlp_features(int n, const float *__restrict__ pos, const float *__restrict__ vel,
                    const int *__restrict__ cs, const float *__restrict__ bpos,
                    const int *__restrict__ bcs, DevGrid g, MlpConst c,
-                   uint16_t *__restrict__ feat, int *__restrict__ flags, int *__restrict__ knn) {
+                   uint16_t *__restrict__ feat, int *__restrict__ flags, int *__restrict__ knn,
+                   int *__restrict__ fb_list, int *__restrict__ fb_count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float xx = pos[3 * i], xy = pos[3 * i + 1], xz = pos[3 * i + 2];
@@ -112,42 +118,64 @@ __global__ void __launch_bounds__(256)
   }
   if (xy < c.guard_floor || xx < c.guard_side || xz < c.guard_side) flag |= 2;
   flags[i] = flag;
+  if (flag && fb_list) {
+    // compacted list of flagged particles (order is irrelevant: each one
+    // writes only its own outputs)
+    const unsigned long long m = __ballot(1);
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int slot = 0;
+    if (lane == leader) slot = atomicAdd(fb_count, __popcll(m));
+    slot = __shfl(slot, leader, 64) + __popcll(m & ((1ull << lane) - 1ull));
+    fb_list[slot] = i;
+  }
   const float vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
-  uint16_t *f = feat + (size_t)i * kIn;
+  const float inv_r = 1.0f / c.radius, inv_v = 1.0f / c.vscale;
+  // two neighbour slots = 12 features = 3 groups of 4
 #pragma unroll
-  for (int s = 0; s < kKnn; ++s) {
-    const int j = kj[s];
-    if (knn) knn[(size_t)i * kKnn + s] = j;
-    uint16_t h[6] = {0, 0, 0, 0, 0, 0};
-    if (j >= 0) {
-      h[0] = to_bf16((pos[3 * j] - xx) / c.radius);
-      h[1] = to_bf16((pos[3 * j + 1] - xy) / c.radius);
-      h[2] = to_bf16((pos[3 * j + 2] - xz) / c.radius);
-      h[3] = to_bf16((vel[3 * j] - vx) / c.vscale);
-      h[4] = to_bf16((vel[3 * j + 1] - vy) / c.vscale);
-      h[5] = to_bf16((vel[3 * j + 2] - vz) / c.vscale);
+  for (int s = 0; s < kKnn; s += 2) {
+    uint16_t h[12];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = kj[s + u];
+      if (knn) knn[(size_t)i * kKnn + s + u] = j;
+      if (j >= 0) {
+        h[6 * u + 0] = to_bf16((pos[3 * j] - xx) * inv_r);
+        h[6 * u + 1] = to_bf16((pos[3 * j + 1] - xy) * inv_r);
+        h[6 * u + 2] = to_bf16((pos[3 * j + 2] - xz) * inv_r);
+        h[6 * u + 3] = to_bf16((vel[3 * j] - vx) * inv_v);
+        h[6 * u + 4] = to_bf16((vel[3 * j + 1] - vy) * inv_v);
+        h[6 * u + 5] = to_bf16((vel[3 * j + 2] - vz) * inv_v);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 6; ++e) h[6 * u + e] = 0;
+      }
     }
-    // 6 halfwords = 3 words at an even halfword offset (96 per particle)
-    uint32_t *fw = reinterpret_cast<uint32_t *>(f + 6 * s);
-    fw[0] = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-    fw[1] = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-    fw[2] = (uint32_t)h[4] | ((uint32_t)h[5] << 16);
+#pragma unroll
+    for (int gq = 0; gq < 3; ++gq) {
+      const uint2 w = make_uint2((uint32_t)h[4 * gq] | ((uint32_t)h[4 * gq + 1] << 16),
+                                 (uint32_t)h[4 * gq + 2] | ((uint32_t)h[4 * gq + 3] << 16));
+      const int grp = 3 * (s / 2) + gq;  // features 4 grp .. 4 grp + 3
+      if (GROUPED) {
+        reinterpret_cast<uint2 *>(feat)[(size_t)grp * n + i] = w;
+      } else {
+        reinterpret_cast<uint2 *>(feat + (size_t)i * kIn)[grp] = w;
+      }
+    }
   }
 }
 
 // ---- teacher ------------------------------------------------------------------
 // One thread per (coarse particle, template slot).  only_flagged: skip the
 // particles whose flag is 0 (the inference fallback).
-__global__ void __launch_bounds__(256)
-    k_mlp_teacher(int n, const float *__restrict__ pos, const float *__restrict__ vel,
-                  const int *__restrict__ cs, const float *__restrict__ bpos,
-                  const int *__restrict__ bcs, DevGrid g, UpConst u, float radius,
-                  const int *__restrict__ flags, float *__restrict__ y,
-                  float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 8LL * n) return;
-  const int i = (int)(t >> 3), k = (int)(t & 7);
-  if (flags && flags[i] == 0) return;
+__device__ __forceinline__ void teacher_slot(int i, int k, const float *__restrict__ pos,
+                                             const float *__restrict__ vel,
+                                             const int *__restrict__ cs,
+                                             const float *__restrict__ bpos,
+                                             const int *__restrict__ bcs, const DevGrid &g,
+                                             const UpConst &u, float radius,
+                                             float *__restrict__ y, float *__restrict__ fine_pos,
+                                             float *__restrict__ fine_vel) {
   const float vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
   const float px = pos[3 * i] + radius * templ(k, 0);
   const float py = pos[3 * i + 1] + radius * templ(k, 1);
@@ -170,6 +198,36 @@ __global__ void __launch_bounds__(256)
     fine_vel[o + 1] = wy;
     fine_vel[o + 2] = wz;
   }
+}
+
+// Fallback: one thread per (listed particle, slot), grid-stride over the
+// device-side count of the compacted flagged list.
+__global__ void __launch_bounds__(256)
+    k_mlp_fallback(const int *__restrict__ list, const int *__restrict__ count,
+                   const float *__restrict__ pos, const float *__restrict__ vel,
+                   const int *__restrict__ cs, const float *__restrict__ bpos,
+                   const int *__restrict__ bcs, DevGrid g, UpConst u, float radius,
+                   float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
+  const long long total = 8LL * (*count);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x)
+    teacher_slot(list[t >> 3], (int)(t & 7), pos, vel, cs, bpos, bcs, g, u, radius, nullptr,
+                 fine_pos, fine_vel);
+}
+
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+__global__ void __launch_bounds__(256)
+    k_mlp_teacher(int n, const float *__restrict__ pos, const float *__restrict__ vel,
+                  const int *__restrict__ cs, const float *__restrict__ bpos,
+                  const int *__restrict__ bcs, DevGrid g, UpConst u, float radius,
+                  const int *__restrict__ flags, float *__restrict__ y,
+                  float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8LL * n) return;
+  const int i = (int)(t >> 3), k = (int)(t & 7);
+  if (flags && flags[i] == 0) return;
+  teacher_slot(i, k, pos, vel, cs, bpos, bcs, g, u, radius, y, fine_pos, fine_vel);
 }
 
 // ---- training: generic bf16 GEMM (fp32 out) ---------------------------------
@@ -289,37 +347,39 @@ __global__ void k_to_bf16(long long count, const float *__restrict__ x, uint16_t
 }
 
 // ---- inference ----------------------------------------------------------------
-// Persistent blocks of 256 threads (4 waves); the weights (bf16, 96 KB) and
-// biases live in LDS.  A wave takes 32 samples per iteration (two 16-sample
-// MFMA column tiles that share every weight fragment).
-constexpr int kInferWaves = 4;
-constexpr int kSampTiles = 2;
+// Persistent blocks of WAVES waves; the weights (bfloat16, 96 KB) and biases
+// live in LDS, shared by the waves.  A wave takes 16 TILES samples per
+// iteration: TILES MFMA column tiles that share every weight fragment.
 
-// One layer, transposed: output tile t (features 16t..16t+15) of both
-// sample tiles from KSTEPS input k-steps; bias, ReLU and bfloat16 packing
+// One layer, transposed: output tile t (features 16t..16t+15) of every
+// sample tile from KSTEPS input k-steps; bias, ReLU and bfloat16 packing
 // give the next layer's B operand directly.
-template <int KSTEPS>
+template <int TILES, int KSTEPS>
 __device__ __forceinline__ void hidden_layer(const uint16_t *__restrict__ W,
                                              const float *__restrict__ bias,
-                                             const bf16x4 (&in)[kSampTiles][8],
-                                             bf16x4 (&out)[kSampTiles][8], int lane) {
+                                             const bf16x4 (&in)[TILES][8],
+                                             bf16x4 (&out)[TILES][8], int lane) {
   const int c = lane & 15, q = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    f32x4 a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+    f32x4 a[TILES];
+#pragma unroll
+    for (int s = 0; s < TILES; ++s) a[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ++ks) {
       const bf16x4 w = *reinterpret_cast<const bf16x4 *>(W + (16 * t + c) * (16 * KSTEPS) +
                                                          16 * ks + 4 * q);
-      a0 = mfma16(w, in[0][ks], a0);
-      a1 = mfma16(w, in[1][ks], a1);
+#pragma unroll
+      for (int s = 0; s < TILES; ++s) a[s] = mfma16(w, in[s][ks], a[s]);
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const float b = bias[16 * t + 4 * q + e];
-      const float z0 = a0[e] + b, z1 = a1[e] + b;
-      out[0][t][e] = (short)to_bf16(z0 > 0.0f ? z0 : 0.0f);
-      out[1][t][e] = (short)to_bf16(z1 > 0.0f ? z1 : 0.0f);
+#pragma unroll
+      for (int s = 0; s < TILES; ++s) {
+        const float z = a[s][e] + b;
+        out[s][t][e] = (short)to_bf16(z > 0.0f ? z : 0.0f);
+      }
     }
     // keep each tile's weight loads next to its MFMAs (bounds live registers)
     __builtin_amdgcn_sched_barrier(0);
@@ -327,31 +387,36 @@ __device__ __forceinline__ void hidden_layer(const uint16_t *__restrict__ W,
 }
 
 // Output layer: 2 tiles (24 outputs + 8 zero rows), no activation.
+template <int TILES>
 __device__ __forceinline__ void output_layer(const uint16_t *__restrict__ W,
                                              const float *__restrict__ bias,
-                                             const bf16x4 (&in)[kSampTiles][8],
-                                             f32x4 (&y)[kSampTiles][2], int lane) {
+                                             const bf16x4 (&in)[TILES][8],
+                                             f32x4 (&y)[TILES][2], int lane) {
   const int c = lane & 15, q = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    f32x4 a0 = {0.0f, 0.0f, 0.0f, 0.0f}, a1 = {0.0f, 0.0f, 0.0f, 0.0f};
+    f32x4 a[TILES];
+#pragma unroll
+    for (int s = 0; s < TILES; ++s) a[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int ks = 0; ks < 8; ++ks) {
       const bf16x4 w = *reinterpret_cast<const bf16x4 *>(W + (16 * t + c) * 128 + 16 * ks + 4 * q);
-      a0 = mfma16(w, in[0][ks], a0);
-      a1 = mfma16(w, in[1][ks], a1);
+#pragma unroll
+      for (int s = 0; s < TILES; ++s) a[s] = mfma16(w, in[s][ks], a[s]);
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const float b = bias[16 * t + 4 * q + e];
-      y[0][t][e] = a0[e] + b;
-      y[1][t][e] = a1[e] + b;
+#pragma unroll
+      for (int s = 0; s < TILES; ++s) y[s][t][e] = a[s][e] + b;
     }
   }
 }
 
-__global__ void __launch_bounds__(256)
-    k_mlp_infer(int n, const uint16_t *__restrict__ feat, const int *__restrict__ flags,
+// feat4: grouped features, feat4[g * n + i] = features 4g..4g+3 of sample i.
+template <int WAVES, int TILES>
+__global__ void __launch_bounds__(64 * WAVES)
+    k_mlp_infer(int n, const uint2 *__restrict__ feat4, const int *__restrict__ flags,
                 const float *__restrict__ pos, const float *__restrict__ vel,
                 const uint16_t *__restrict__ wts, const float *__restrict__ bias_g, float radius,
                 float dt, float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
@@ -363,34 +428,33 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = lane & 15, q = lane >> 4;
-  const int per_iter = 16 * kSampTiles;
+  const int per_iter = 16 * TILES;
   const int groups = (n + per_iter - 1) / per_iter;
-  for (int grp = blockIdx.x * kInferWaves + wave; grp < groups; grp += gridDim.x * kInferWaves) {
+  for (int grp = blockIdx.x * WAVES + wave; grp < groups; grp += gridDim.x * WAVES) {
     const int base = grp * per_iter;
-    bf16x4 h[kSampTiles][8], h2[kSampTiles][8];
-    f32x4 y[kSampTiles][2];
-    // layer 1 input: features k = 16 ks + 4 q + e of sample base + 16 s + c
+    bf16x4 h[TILES][8], h2[TILES][8];
+    f32x4 y[TILES][2];
+    // layer 1 input: features 16 ks + 4 q + e of sample base + 16 s + c
 #pragma unroll
-    for (int s = 0; s < kSampTiles; ++s) {
+    for (int s = 0; s < TILES; ++s) {
       const int smp = base + 16 * s + c;
 #pragma unroll
       for (int ks = 0; ks < 6; ++ks) {
-        if (smp < n) {
-          h[s][ks] = *reinterpret_cast<const bf16x4 *>(feat + (size_t)smp * kIn + 16 * ks + 4 * q);
-        } else {
-          h[s][ks] = bf16x4{0, 0, 0, 0};
-        }
+        uint2 w = make_uint2(0u, 0u);
+        if (smp < n) w = feat4[(size_t)(4 * ks + q) * n + smp];
+        h[s][ks] = bf16x4{(short)(w.x & 0xffffu), (short)(w.x >> 16), (short)(w.y & 0xffffu),
+                          (short)(w.y >> 16)};
       }
       h[s][6] = bf16x4{0, 0, 0, 0};
       h[s][7] = bf16x4{0, 0, 0, 0};
     }
-    hidden_layer<6>(W + kWOff1, bias + kBOff1, h, h2, lane);
-    hidden_layer<8>(W + kWOff2, bias + kBOff2, h2, h, lane);
-    hidden_layer<8>(W + kWOff3, bias + kBOff3, h, h2, lane);
-    output_layer(W + kWOff4, bias + kBOff4, h2, y, lane);
+    hidden_layer<TILES, 6>(W + kWOff1, bias + kBOff1, h, h2, lane);
+    hidden_layer<TILES, 8>(W + kWOff2, bias + kBOff2, h2, h, lane);
+    hidden_layer<TILES, 8>(W + kWOff3, bias + kBOff3, h, h2, lane);
+    output_layer<TILES>(W + kWOff4, bias + kBOff4, h2, y, lane);
     // outputs f = 16 t + 4 q + e of sample base + 16 s + c
 #pragma unroll
-    for (int s = 0; s < kSampTiles; ++s) {
+    for (int s = 0; s < TILES; ++s) {
       const int smp = base + 16 * s + c;
       if (smp >= n || flags[smp] != 0) continue;
       const float px = pos[3 * smp], py = pos[3 * smp + 1], pz = pos[3 * smp + 2];
@@ -515,6 +579,8 @@ extern "C" void gg_mlp_config_default(gg_mlp_config *c) {
   c->guard_boundary = 0.05f;
   c->guard_floor = 0.08f;
   c->guard_side = 0.05f;
+  c->infer_waves = 0;
+  c->infer_tiles = 0;
 }
 
 extern "C" int gg_mlp_features(const gg_particles *coarse, const int *cs,
@@ -526,11 +592,11 @@ extern "C" int gg_mlp_features(const gg_particles *coarse, const int *cs,
   if (!(c->radius > 0.0f) || !(c->vscale > 0.0f)) return GG_ERR_INVALID;
   if (coarse->n == 0) return GG_OK;
   if (!coarse->pos || !coarse->vel || (boundary->n > 0 && !boundary->pos)) return GG_ERR_INVALID;
-  k_mlp_features<<<nblocks(coarse->n, 256), 256, 0, stream>>>(
+  k_mlp_features<0><<<nblocks(coarse->n, 256), 256, 0, stream>>>(
       coarse->n, reinterpret_cast<const float *>(coarse->pos),
       reinterpret_cast<const float *>(coarse->vel), cs,
       reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), mlp_const(c), feat,
-      flags, knn);
+      flags, knn, nullptr, nullptr);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
@@ -634,7 +700,20 @@ extern "C" size_t gg_upsample_mlp_scratch_bytes(int n) {
   if (n < 0) return 0;
   const size_t feat = ((size_t)n * kIn * 2 + 255) & ~(size_t)255;
   const size_t fl = ((size_t)n * 4 + 255) & ~(size_t)255;
-  return feat + fl + 256;
+  return feat + 2 * fl + 256;
+}
+
+template <int WAVES, int TILES>
+static int launch_infer(int n, const uint2 *feat4, const int *flags, const float *pos,
+                        const float *vel, const uint16_t *w, const float *bias, float radius,
+                        float dt, float *fine_pos, float *fine_vel, int cus, hipStream_t st) {
+  const int groups = (n + 16 * TILES - 1) / (16 * TILES);
+  int blocks = (groups + WAVES - 1) / WAVES;
+  if (blocks > cus) blocks = cus;
+  k_mlp_infer<WAVES, TILES><<<blocks, 64 * WAVES, 0, st>>>(n, feat4, flags, pos, vel, w, bias,
+                                                          radius, dt, fine_pos, fine_vel);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
 }
 
 extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
@@ -647,33 +726,59 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
       !scratch)
     return GG_ERR_INVALID;
   const int n = coarse->n;
-  if (n < 0) return GG_ERR_INVALID;
+  if (n < 0 || grid.cx <= 0 || grid.cy <= 0 || grid.cz <= 0) return GG_ERR_INVALID;
+  if (!(c->radius > 0.0f) || !(c->vscale > 0.0f)) return GG_ERR_INVALID;
+  const int waves = c->infer_waves == 0 ? 8 : c->infer_waves;
+  const int tiles = c->infer_tiles == 0 ? 4 : c->infer_tiles;
+  // (16 waves, 2 or 4 tiles) would spill registers: not offered
+  if (!(((waves == 4 || waves == 8) && (tiles == 1 || tiles == 2 || tiles == 4)) ||
+        (waves == 16 && tiles == 1)))
+    return GG_ERR_INVALID;
   if (scratch_bytes < gg_upsample_mlp_scratch_bytes(n)) return GG_ERR_WORKSPACE;
   if (n == 0) return GG_OK;
+  if (!coarse->pos || !coarse->vel || (boundary->n > 0 && !boundary->pos)) return GG_ERR_INVALID;
   uint16_t *feat = static_cast<uint16_t *>(scratch);
+  const size_t fl_bytes = ((size_t)n * 4 + 255) & ~(size_t)255;
   int *flags = reinterpret_cast<int *>(static_cast<char *>(scratch) +
                                        (((size_t)n * kIn * 2 + 255) & ~(size_t)255));
+  int *fb_list = reinterpret_cast<int *>(reinterpret_cast<char *>(flags) + fl_bytes);
+  int *fb_count = reinterpret_cast<int *>(reinterpret_cast<char *>(fb_list) + fl_bytes);
+  const float *pos = reinterpret_cast<const float *>(coarse->pos);
+  const float *vel = reinterpret_cast<const float *>(coarse->vel);
+  const DevGrid g = dev_grid(grid);
   prof_begin(GG_STAGE_MLP_FEATURES, stream);
-  GG_TRY(gg_mlp_features(coarse, cs, boundary, bcs, grid, c, feat, flags, nullptr, stream));
+  k_zero_int<<<1, 1, 0, stream>>>(fb_count);
+  GG_LAUNCH_CHECK();
+  k_mlp_features<1><<<nblocks(n, 256), 256, 0, stream>>>(
+      n, pos, vel, cs, reinterpret_cast<const float *>(boundary->pos), bcs, g, mlp_const(c), feat,
+      flags, nullptr, fb_list, fb_count);
+  GG_LAUNCH_CHECK();
   prof_end(GG_STAGE_MLP_FEATURES, stream);
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const int groups = (n + 16 * kSampTiles - 1) / (16 * kSampTiles);
-  int grid_blocks = (groups + kInferWaves - 1) / kInferWaves;
-  if (grid_blocks > cus) grid_blocks = cus;
+  const uint2 *feat4 = reinterpret_cast<const uint2 *>(feat);
   prof_begin(GG_STAGE_MLP, stream);
-  k_mlp_infer<<<grid_blocks, 256, 0, stream>>>(n, feat, flags,
-                                               reinterpret_cast<const float *>(coarse->pos),
-                                               reinterpret_cast<const float *>(coarse->vel),
-                                               w_bf16, bias, c->radius, u->dt, fine_pos,
-                                               fine_vel);
-  GG_LAUNCH_CHECK();
+  int rc;
+#define GG_INFER(W_, T_)                                                                     \
+  rc = launch_infer<W_, T_>(n, feat4, flags, pos, vel, w_bf16, bias, c->radius, u->dt,       \
+                            fine_pos, fine_vel, cus, stream)
+  if (waves == 4 && tiles == 1) GG_INFER(4, 1);
+  else if (waves == 4 && tiles == 2) GG_INFER(4, 2);
+  else if (waves == 4 && tiles == 4) GG_INFER(4, 4);
+  else if (waves == 8 && tiles == 1) GG_INFER(8, 1);
+  else if (waves == 8 && tiles == 2) GG_INFER(8, 2);
+  else if (waves == 8 && tiles == 4) GG_INFER(8, 4);
+  else GG_INFER(16, 1);
+#undef GG_INFER
+  if (rc != GG_OK) return rc;
   prof_end(GG_STAGE_MLP, stream);
   // flagged particles: the teacher (the plain upsampler on the template)
   prof_begin(GG_STAGE_MLP_FALLBACK, stream);
-  GG_TRY(gg_mlp_teacher(coarse, cs, boundary, bcs, grid, u, c, flags, nullptr, fine_pos,
-                        fine_vel, stream));
+  k_mlp_fallback<<<cus * 8, 256, 0, stream>>>(
+      fb_list, fb_count, pos, vel, cs, reinterpret_cast<const float *>(boundary->pos), bcs, g,
+      up_const(u), c->radius, fine_pos, fine_vel);
+  GG_LAUNCH_CHECK();
   prof_end(GG_STAGE_MLP_FALLBACK, stream);
   (void)fallback_count;
   return GG_OK;

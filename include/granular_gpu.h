@@ -289,8 +289,8 @@ const void *gg_workspace_view(const gg_workspace *ws, int which);
  * update_upsampled_cuda, Solver.cu:1244-1387).  Definitions:
  * - features of coarse particle i: its 16 nearest coarse particles j != i
  *   in the 27-cell stencil of floor(x_i/L), ordered by (|x_j - x_i|^2, j);
- *   slot s = ((x_j - x_i)/radius, (v_j - v_i)/vscale), zeros when fewer
- *   than 16; 96 bfloat16 values;
+ *   slot s = ((x_j - x_i) fl(1/radius), (v_j - v_i) fl(1/vscale)), zeros
+ *   when fewer than 16; 96 bfloat16 values;
  * - flags: bit 0 = a boundary particle closer than guard_boundary; bit 1 =
  *   y < guard_floor or x, z < guard_side.  Flagged particles take the
  *   teacher's result (the fallback);
@@ -311,6 +311,9 @@ typedef struct gg_mlp_config {
   float guard_boundary;  /* 0.05                                  */
   float guard_floor;     /* 0.08                                  */
   float guard_side;      /* 0.05                                  */
+  int infer_waves;       /* inference: waves per block, 4, 8 or 16 (0 = 8)  */
+  int infer_tiles;       /* inference: 16-sample tiles per wave, 1, 2 or 4
+                            (0 = 4); 16 waves only with 1 tile             */
 } gg_mlp_config;
 typedef struct gg_mlp_weights {
   uint16_t *w_bf16;  /* GG_MLP_WEIGHTS bfloat16 (inference and training)  */

@@ -5,9 +5,10 @@
 //
 // Features of coarse particle i: the 16 coarse particles j != i of the
 // 27-cell stencil of floor(x_i/L) with the smallest (|x_j - x_i|^2, j),
-// ascending; slot s holds ((x_j - x_i)/R, (v_j - v_i)/V) in the particle's
-// co-moving frame (R = coarse radius, V = velocity scale); empty slots are
-// zero; every feature is rounded to bfloat16.
+// ascending; slot s holds ((x_j - x_i) fl(1/R), (v_j - v_i) fl(1/V)) in the
+// particle's co-moving frame (R = coarse radius, V = velocity scale, one
+// rounded reciprocal each); empty slots are zero; every feature is rounded
+// to bfloat16.
 //
 // Fallback flags: bit 0 = a boundary particle of the stencil closer than
 // guard_boundary; bit 1 = y_i < guard_floor, x_i < guard_side or
@@ -147,6 +148,7 @@ extern "C" int oracle_mlp_features(int n, const float *pos_, const float *vel_, 
     }
     if (xi.y < c->guard_floor || xi.x < c->guard_side || xi.z < c->guard_side) flag |= 2;
     float *f = feat + (size_t)i * kIn;
+    const float inv_r = 1.0f / c->radius, inv_v = 1.0f / c->vscale;
     for (int s = 0; s < kKnn; ++s) {
       const int j = kj[s];
       if (knn) knn[(size_t)i * kKnn + s] = j;
@@ -154,12 +156,12 @@ extern "C" int oracle_mlp_features(int n, const float *pos_, const float *vel_, 
         for (int q = 0; q < 6; ++q) f[6 * s + q] = 0.0f;
         continue;
       }
-      f[6 * s + 0] = bf16r((pos[j].x - xi.x) / c->radius);
-      f[6 * s + 1] = bf16r((pos[j].y - xi.y) / c->radius);
-      f[6 * s + 2] = bf16r((pos[j].z - xi.z) / c->radius);
-      f[6 * s + 3] = bf16r((vel[j].x - vi.x) / c->vscale);
-      f[6 * s + 4] = bf16r((vel[j].y - vi.y) / c->vscale);
-      f[6 * s + 5] = bf16r((vel[j].z - vi.z) / c->vscale);
+      f[6 * s + 0] = bf16r((pos[j].x - xi.x) * inv_r);
+      f[6 * s + 1] = bf16r((pos[j].y - xi.y) * inv_r);
+      f[6 * s + 2] = bf16r((pos[j].z - xi.z) * inv_r);
+      f[6 * s + 3] = bf16r((vel[j].x - vi.x) * inv_v);
+      f[6 * s + 4] = bf16r((vel[j].y - vi.y) * inv_v);
+      f[6 * s + 5] = bf16r((vel[j].z - vi.z) * inv_v);
     }
     flags[i] = flag;
   }

@@ -159,6 +159,20 @@ def test_inference_matches_training_forward_and_fallback(gpu):
     assert d < 1e-5
 
 
+def test_inference_configs_agree(gpu):
+    # waves per block and sample tiles per wave change the launch shape only
+    from granular_gpu_amd.learned import LearnedUpsampler, MlpNet
+    s, sim = _pile()
+    up = LearnedUpsampler(MlpNet(seed=7))
+    ref = None
+    for w, t in [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (8, 4), (16, 1)]:
+        up.cfg.infer_waves, up.cfg.infer_tiles = w, t
+        got = up.infer(sim).cpu().numpy()
+        if ref is None:
+            ref = got
+        assert np.array_equal(got, ref), (w, t)
+
+
 def test_training_gains_over_rigid_template(gpu):
     # train on snapshots of one run, judge on another seed's dynamic phase
     from granular_gpu_amd.device import PlainSim

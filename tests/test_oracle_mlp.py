@@ -73,8 +73,8 @@ def test_features_match_brute_force():
         for slot, j in enumerate(want):
             if j < 0:
                 continue
-            f[slot, :3] = (pos[j] - pos[i]) / np.float32(R)
-            f[slot, 3:] = (vel[j] - vel[i]) / np.float32(1.0)
+            f[slot, :3] = (pos[j] - pos[i]) * (np.float32(1.0) / np.float32(R))
+            f[slot, 3:] = (vel[j] - vel[i]) * (np.float32(1.0) / np.float32(1.0))
         assert np.array_equal(feat[i], bf16_np(f.reshape(-1))), i
     assert (flags == 0).all()
 
