@@ -116,6 +116,16 @@ def upsample_phase(args, lib, torch, C, np):
     up_us, sort_us = st["upsample"]["avg_us"], st["upsample_sort"]["avg_us"]
     per_launch = 48 * nf + 24 * n + 12 * nb + 8 * cells
     ach = per_launch / (up_us * 1e-6) / 1e9
+    pmc_up = {}
+    pmc_path = ROOT / "profiles" / "pmc_c3_upsample_learned.json"
+    if pmc_path.exists():
+        try:
+            pmc_up = json.loads(pmc_path.read_text()).get("kernels", {})
+        except Exception:
+            pmc_up = {}
+
+    def traffic_of(kernel):
+        return pmc_up.get(kernel, {}).get("hbm_bytes_per_launch")
     print(f"# upsample timed {args.upsample_steps} steps in {el:.3f} s", file=sys.stderr,
           flush=True)
     learned_out = None
@@ -145,6 +155,7 @@ def upsample_phase(args, lib, torch, C, np):
             "roofline": {"kernel": "k_mlp_infer (fused MFMA network)", "bound": "mfma",
                          "achieved": round(ach, 2), "peak": 2500.0, "unit": "TFLOP/s",
                          "frac": round(ach / 2500.0, 4),
+                         "traffic": traffic_of("void gg::k_mlp_infer<8, 4>"),
                          "algorithmic_flops_per_launch": flops, "avg_launch_us": t_net},
         }
         print(f"# learned: {1e-9 * nf / (t_all * 1e-6):.2f} G fine/s, error {e_l:.4f} r_fine "
@@ -161,7 +172,7 @@ def upsample_phase(args, lib, torch, C, np):
         "stages": st,
         "roofline": {"kernel": "k_upsample (plain upsampler pass)", "bound": "hbm",
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_of("gg::k_upsample"),
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": up_us},
         "learned": learned_out,
     }
@@ -234,6 +245,8 @@ def main():
     ap.add_argument("--upsample-side", type=int, default=63, help="63 -> 250,047 coarse")
     ap.add_argument("--upsample-steps", type=int, default=100)
     ap.add_argument("--upsample-warmup", type=int, default=20)
+    ap.add_argument("--only", choices=["upsample", "batch"], default=None,
+                    help="run one secondary phase alone and print its object (profiling)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--learned-train-steps", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true")
@@ -278,6 +291,13 @@ def main():
     if lib.gg_device_check() != 0:
         print(json.dumps({"error": "device is not gfx950"}), flush=True)
         sys.exit(1)
+
+    if args.only in ("upsample", "batch"):
+        # one secondary phase alone (for profiling it in isolation)
+        fn = upsample_phase if args.only == "upsample" else batch_phase
+        print(json.dumps({"only": args.only, args.only: fn(args, lib, torch, C, np)}),
+              flush=True)
+        return
 
     S, I, mu_s, mu_k = 3, 10, 0.5, 0.4
     scene = scenes.plain_box(args.side, args.box, dt=args.dt, g=-9.8)

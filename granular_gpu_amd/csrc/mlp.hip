@@ -315,13 +315,22 @@ __global__ void k_mask(long long count, const float *__restrict__ D, const float
   G[t] = to_bf16(v);
 }
 
-// db[n] = sum over b of g[b][n], in sample order (deterministic).
-__global__ void k_colsum(int B, int N, const float *__restrict__ g, float *__restrict__ db) {
-  const int nn = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nn >= N) return;
+// db[n] = sum over b of g[b][n]: one block per column; thread t sums
+// samples t, t + 256, ... in order, then a fixed tree in LDS (the same
+// order on every run).
+__global__ void __launch_bounds__(256) k_colsum(int B, int N, const float *__restrict__ g,
+                                                float *__restrict__ db) {
+  __shared__ float part[256];
+  const int nn = blockIdx.x;
   float s = 0.0f;
-  for (int b = 0; b < B; ++b) s = s + g[(size_t)b * N + nn];
-  db[nn] = s;
+  for (int b = threadIdx.x; b < B; b += 256) s = s + g[(size_t)b * N + nn];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] = part[threadIdx.x] + part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[nn] = part[0];
 }
 
 __global__ void k_adam(int count, float *__restrict__ w, float *__restrict__ m,
@@ -653,7 +662,7 @@ extern "C" int gg_mlp_grad(const gg_mlp_weights *w, const uint16_t *x, const flo
     const uint16_t *hin = (l == 0) ? x : s.H[l - 1];
     // dW_l[n][k] = sum_b G[b][n] h[b][k]
     GG_TRY(gemm(s.G, 1, N, hin, K, 1, dW + kWOffs[l], K, N, K, B, stream));
-    k_colsum<<<nblocks(N, 128), 128, 0, stream>>>(B, N, g, db + kBOffs[l]);
+    k_colsum<<<N, 256, 0, stream>>>(B, N, g, db + kBOffs[l]);
     GG_LAUNCH_CHECK();
     if (l == 0) break;
     // D[b][k] = sum_n G[b][n] W_l[n][k]; g_prev = D * [Z_{l-1} > 0]
