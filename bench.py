@@ -142,7 +142,7 @@ def upsample_phase(args, lib, torch, C, np):
         t_fb = st["mlp_fallback"]["avg_us"]
         t_all = t_net + t_feat + t_fb
         flops = 96256 * n
-        ach = flops / (t_net * 1e-6) / 1e12
+        ach_l = flops / (t_net * 1e-6) / 1e12
         learned_out = {
             "network": "96-128-128-128-24 ReLU, bf16 weights/activations, fp32 accumulate",
             "training": train,
@@ -153,8 +153,8 @@ def upsample_phase(args, lib, torch, C, np):
             "error_vs_teacher_r_fine": round(e_l, 5),
             "rigid_template_error_r_fine": round(e_r, 5),
             "roofline": {"kernel": "k_mlp_infer (fused MFMA network)", "bound": "mfma",
-                         "achieved": round(ach, 2), "peak": 2500.0, "unit": "TFLOP/s",
-                         "frac": round(ach / 2500.0, 4),
+                         "achieved": round(ach_l, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(ach_l / 2500.0, 4),
                          "traffic": traffic_of("void gg::k_mlp_infer<8, 4>"),
                          "algorithmic_flops_per_launch": flops, "avg_launch_us": t_net},
         }
