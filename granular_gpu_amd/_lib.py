@@ -20,6 +20,9 @@ GG_ERR_HIP = -3
 GG_ERR_WORKSPACE = -4
 GG_ERR_NO_DEVICE = -5
 GG_FLAG_NO_NLIST = 1
+GG_FLAG_NO_XCD_REMAP = 2
+GG_FLAG_FALLBACK_LANE = 4
+GG_FLAG_FALLBACK_WAVE = 8
 STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep", "adaptive", "upsample",
           "upsample_sort", "mlp", "mlp_features", "mlp_fallback"]
 

@@ -138,31 +138,73 @@ __device__ __forceinline__ float3 friction(float px, float py, float pz, float i
 // root when it fails cannot change any result.
 __device__ __forceinline__ float contact_prefilter(float rc) { return rc * rc * 1.000001f; }
 
+// Boundary contact terms of particle (bi, oi) with static boundary particle
+// q: the correction c = s p and the friction f.  False when not in contact.
+template <bool FRIC>
+__device__ __forceinline__ bool boundary_terms(const float4 &bi, const float4 &oi, const float4 &q,
+                                               float rc, float pre, const SweepConst &sc,
+                                               float3 &c, float3 &f) {
+  const float px = (bi.x - q.x) + oi.x;
+  const float py = (bi.y - q.y) + oi.y;
+  const float pz = (bi.z - q.z) + oi.z;
+  const float d2 = px * px + py * py + pz * pz;
+  if (!(d2 <= pre)) return false;
+  const float d = sqrtf(d2);
+  if (!(d < rc && d > 0.0f)) return false;
+  const float inv = 1.0f / d;
+  const float s = (rc - d) * inv;
+  c = make_float3(s * px, s * py, s * pz);
+  if (FRIC) f = friction(px, py, pz, inv, rc - d, oi.x, oi.y, oi.z, sc.mu_s, sc.mu_k);
+  return true;
+}
+
+// Granular contact terms of particle i with particle j (symmetric gather).
+template <bool FRIC>
+__device__ __forceinline__ bool granular_terms(const float4 &bi, const float4 &oi, const float4 &bj,
+                                               const float4 &oj, const SweepConst &sc, float3 &c,
+                                               float3 &f) {
+  const float px = (bi.x - bj.x) + (oi.x - oj.x);
+  const float py = (bi.y - bj.y) + (oi.y - oj.y);
+  const float pz = (bi.z - bj.z) + (oi.z - oj.z);
+  const float d2 = px * px + py * py + pz * pz;
+  const float rc = bi.w + bj.w;
+  if (!(d2 <= contact_prefilter(rc))) return false;
+  const float d = sqrtf(d2);
+  if (!(d <= rc && d > 0.0f)) return false;
+  const float inv = 1.0f / d;
+  const float s = (rc - d) * inv;
+  c = make_float3(s * px, s * py, s * pz);
+  if (FRIC) {
+    const float rx = oi.x - oj.x;
+    const float ry = oi.y - oj.y;
+    const float rz = oi.z - oj.z;
+    f = friction(px, py, pz, inv, rc - d, rx, ry, rz, sc.mu_s, sc.mu_k);
+  }
+  return true;
+}
+
+// Adds one contact's terms: a + c, then - f (the order every path uses).
+template <bool FRIC>
+__device__ __forceinline__ void add_terms(float3 &a, const float3 &c, const float3 &f) {
+  a.x = a.x + c.x;
+  a.y = a.y + c.y;
+  a.z = a.z + c.z;
+  if (FRIC) {
+    a.x = a.x - f.x;
+    a.y = a.y - f.y;
+    a.z = a.z - f.z;
+  }
+}
+
 // Boundary contact of particle (bi, oi) with static boundary particle q.
 template <bool FRIC>
 __device__ __forceinline__ void boundary_contact(const float4 &bi, const float4 &oi, const float4 &q,
                                                  float rc, float pre, const SweepConst &sc,
                                                  float3 &ab, int &nb) {
-  const float px = (bi.x - q.x) + oi.x;
-  const float py = (bi.y - q.y) + oi.y;
-  const float pz = (bi.z - q.z) + oi.z;
-  const float d2 = px * px + py * py + pz * pz;
-  if (!(d2 <= pre)) return;
-  const float d = sqrtf(d2);
-  if (d < rc && d > 0.0f) {
-    const float inv = 1.0f / d;
-    const float s = (rc - d) * inv;
-    ab.x = ab.x + s * px;
-    ab.y = ab.y + s * py;
-    ab.z = ab.z + s * pz;
-    if (FRIC) {
-      const float3 f = friction(px, py, pz, inv, rc - d, oi.x, oi.y, oi.z, sc.mu_s, sc.mu_k);
-      ab.x = ab.x - f.x;
-      ab.y = ab.y - f.y;
-      ab.z = ab.z - f.z;
-    }
-    nb += 1;
-  }
+  float3 c, f = make_float3(0.0f, 0.0f, 0.0f);
+  if (!boundary_terms<FRIC>(bi, oi, q, rc, pre, sc, c, f)) return;
+  add_terms<FRIC>(ab, c, f);
+  nb += 1;
 }
 
 // Granular contact of particle i with particle j (symmetric gather).
@@ -170,30 +212,10 @@ template <bool FRIC>
 __device__ __forceinline__ void granular_contact(const float4 &bi, const float4 &oi, const float4 &bj,
                                                  const float4 &oj, const SweepConst &sc, float3 &ag,
                                                  int &ng) {
-  const float px = (bi.x - bj.x) + (oi.x - oj.x);
-  const float py = (bi.y - bj.y) + (oi.y - oj.y);
-  const float pz = (bi.z - bj.z) + (oi.z - oj.z);
-  const float d2 = px * px + py * py + pz * pz;
-  const float rc = bi.w + bj.w;
-  if (!(d2 <= contact_prefilter(rc))) return;
-  const float d = sqrtf(d2);
-  if (d <= rc && d > 0.0f) {
-    const float inv = 1.0f / d;
-    const float s = (rc - d) * inv;
-    ag.x = ag.x + s * px;
-    ag.y = ag.y + s * py;
-    ag.z = ag.z + s * pz;
-    if (FRIC) {
-      const float rx = oi.x - oj.x;
-      const float ry = oi.y - oj.y;
-      const float rz = oi.z - oj.z;
-      const float3 f = friction(px, py, pz, inv, rc - d, rx, ry, rz, sc.mu_s, sc.mu_k);
-      ag.x = ag.x - f.x;
-      ag.y = ag.y - f.y;
-      ag.z = ag.z - f.z;
-    }
-    ng += 2;
-  }
+  float3 c, f = make_float3(0.0f, 0.0f, 0.0f);
+  if (!granular_terms<FRIC>(bi, oi, bj, oj, sc, c, f)) return;
+  add_terms<FRIC>(ag, c, f);
+  ng += 2;
 }
 
 __device__ __forceinline__ float wave_max(float v) {

@@ -21,6 +21,7 @@ constexpr float kPiF = 3.14159265358979323846f;
 // that of every particle of its stencil stay below half of this fraction of
 // the skin.
 constexpr float kListValidFrac = 0.9f;
+constexpr int kFallbackWaveMax = 6;  // fallbacks per wave handed to whole waves (else per lane)
 constexpr int kFallbackSlots = 62;  // ctr[1 + k % 62] = full-stencil count of sweep k
 // ctr[63..] are not used by the sweeps; the adaptive passes use ctr2 (own region)
 constexpr int kMergeRounds = 12;    // parallel claim-resolution rounds before the finisher

@@ -20,9 +20,11 @@
 // particle i's list only when (a) the stencil base of x*_i is the one the
 // list was built for and (b) i's accumulated displacement plus the largest
 // displacement of any particle is below 0.9*skin, which guarantees every
-// pair outside the list is still out of contact.  Otherwise the thread scans
-// the full 27-cell stencil.  Both paths visit contacts in the same order,
-// so the result is identical to always scanning the stencil.
+// pair outside the list is still out of contact.  Otherwise the particle
+// scans the full 27-cell stencil, in its own lane or, when few lanes of its
+// wave need to, by a whole wave after the list work of the block.  All paths
+// visit contacts in the same order, so the result is identical to always
+// scanning the stencil.
 #include <hip/hip_runtime.h>
 
 #include "gg_device.hpp"
@@ -147,6 +149,7 @@ struct SweepArgs {
   SweepConst sc;
   int use_lists;
   int remap;
+  int fb_wave_max;  // a wave hands at most this many fallbacks to whole waves
 };
 
 // Shared epilogue: new offset (and for the last stabilisation sweep the
@@ -209,33 +212,157 @@ __device__ __forceinline__ void full_stencil(const SweepArgs &a, int i, const fl
   });
 }
 
+// Rows of the 27-cell stencil of (bx, by, bz) over cell table cs, held
+// wave-uniform: row r (dx = r/3 - 1, dy = r%3 - 1, the three z cells) is the
+// index range [start[r], start[r] + len[r]).  Rows in scan order.
+struct StencilRows {
+  int start[9], len[9], total;
+};
+
+__device__ __forceinline__ void stencil_row_table(int bx, int by, int bz, const DevGrid &g,
+                                                  const int *__restrict__ cs, int lane,
+                                                  StencilRows &rt) {
+  int s = 0, e = 0;
+  const int z0 = bz - 1 > 0 ? bz - 1 : 0;
+  const int z1 = bz + 1 < g.cz - 1 ? bz + 1 : g.cz - 1;
+  if (lane < 9 && z0 <= z1) {
+    const int x = bx + lane / 3 - 1, y = by + lane % 3 - 1;
+    if (x >= 0 && x < g.cx && y >= 0 && y < g.cy) {
+      const int c = (x * g.cy + y) * g.cz;
+      s = cs[c + z0];
+      e = cs[c + z1 + 1];
+    }
+  }
+  const int len = e - s;
+  rt.total = 0;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    rt.start[r] = __builtin_amdgcn_readlane(s, r);
+    rt.len[r] = __builtin_amdgcn_readlane(len, r);
+    rt.total += rt.len[r];
+  }
+}
+
+// Index of the t-th candidate of the row table in scan order (-1 past the end).
+__device__ __forceinline__ int stencil_row_index(const StencilRows &rt, int t) {
+  int j = -1, cum = 0;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int d = t - cum;
+    if (d >= 0 && d < rt.len[r]) j = rt.start[r] + d;
+    cum += rt.len[r];
+  }
+  return j;
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Adds the terms of the contacts flagged in mask, lowest lane first.
+template <bool FRIC>
+__device__ __forceinline__ void add_lanes(unsigned long long mask, const float3 &c, const float3 &f,
+                                          float3 &acc) {
+  while (mask) {
+    const int l = __ffsll((long long)mask) - 1;
+    mask &= mask - 1;
+    const float3 cl = make_float3(readlane_f(c.x, l), readlane_f(c.y, l), readlane_f(c.z, l));
+    float3 fl = make_float3(0.0f, 0.0f, 0.0f);
+    if (FRIC) fl = make_float3(readlane_f(f.x, l), readlane_f(f.y, l), readlane_f(f.z, l));
+    add_terms<FRIC>(acc, cl, fl);
+  }
+}
+
+// full_stencil for one particle by a whole wave: the lanes evaluate the
+// stencil's candidates 64 at a time in scan order, and the contacts found
+// are added one after another in that order, so the sums equal
+// full_stencil's bit for bit.  Every lane of the wave calls it with the
+// same particle; the accumulators come back wave-uniform.
+template <bool FRIC>
+__device__ __forceinline__ void wave_stencil(const SweepArgs &a, int i, const float4 &bi,
+                                             const float4 &oi, int bx, int by, int bz, float rcb,
+                                             float preb, float3 &ab, float3 &ag, int &nb,
+                                             int &ng) {
+  const SweepConst &sc = a.sc;
+  const int lane = threadIdx.x & 63;
+  StencilRows rt;
+  stencil_row_table(bx, by, bz, a.g, a.bcs, lane, rt);
+  for (int t0 = 0; t0 < rt.total; t0 += 64) {
+    const int k = stencil_row_index(rt, t0 + lane);
+    float3 c = make_float3(0.0f, 0.0f, 0.0f), f = c;
+    bool hit = false;
+    if (k >= 0) hit = boundary_terms<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, c, f);
+    const unsigned long long m = __ballot(hit);
+    nb += __popcll(m);
+    add_lanes<FRIC>(m, c, f, ab);
+  }
+  stencil_row_table(bx, by, bz, a.g, a.cs, lane, rt);
+  for (int t0 = 0; t0 < rt.total; t0 += 64) {
+    const int j = stencil_row_index(rt, t0 + lane);
+    float3 c = make_float3(0.0f, 0.0f, 0.0f), f = c;
+    bool hit = false;
+    if (j >= 0 && j != i) hit = granular_terms<FRIC>(bi, oi, a.base[j], a.off_in[j], sc, c, f);
+    const unsigned long long m = __ballot(hit);
+    ng += 2 * __popcll(m);
+    add_lanes<FRIC>(m, c, f, ag);
+  }
+}
+
 // One sweep.  Particles whose candidate list is provably exact for this
-// sweep use it; the others scan the full stencil.  Both visit contacts in
+// sweep use it; the others scan the full stencil, either in their own lane
+// or (when few lanes of the wave need it) queued in LDS and scanned by a
+// whole wave of the block after the list work.  All paths visit contacts in
 // the same order, so the result does not depend on which path ran.
 template <bool FRIC, bool LAST_STAB, int BS>
 __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
+  __shared__ int queue[BS];
+  __shared__ int queued;
+  const int lane = threadIdx.x & 63;
+  if (a.use_lists) {
+    if (threadIdx.x == 0) queued = 0;
+    __syncthreads();
+  }
   const int i = xcd_block(blockIdx.x, gridDim.x, a.remap) * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
   const DevGrid &g = a.g;
   const SweepConst &sc = a.sc;
-  const float4 bi = a.base[i];
-  const float4 oi = a.off_in[i];
+  // 0 idle lane, 1 list, 2 stencil scan, 3 stencil outside the grid
+  int mode = 0;
+  float4 bi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), oi = bi;
   int bx = 0, by = 0, bz = 0;
-  const bool ok = trunc_cell(bi.x + oi.x, bi.y + oi.y, bi.z + oi.z, g, bx, by, bz);
-  const int ext = ext_base_id(ok, bx, by, bz, g);
-  bool list_ok = false;
   int4 mt = make_int4(0, 0, -1, -1);
-  if (a.use_lists) {
-    mt = a.meta[i];
-    const int bcell = base_cell_id(ok, bx, by, bz, g);
-    list_ok = ext == mt.z && mt.x <= sc.gmax && mt.y <= sc.bmax && oi.w < sc.half_valid &&
-              bcell >= 0 && a.dirty[bcell] == 0;
+  if (i < a.n) {
+    bi = a.base[i];
+    oi = a.off_in[i];
+    const bool ok = trunc_cell(bi.x + oi.x, bi.y + oi.y, bi.z + oi.z, g, bx, by, bz);
+    const int ext = ext_base_id(ok, bx, by, bz, g);
+    bool list_ok = false;
+    if (a.use_lists) {
+      mt = a.meta[i];
+      const int bcell = base_cell_id(ok, bx, by, bz, g);
+      list_ok = ext == mt.z && mt.x <= sc.gmax && mt.y <= sc.bmax && oi.w < sc.half_valid &&
+                bcell >= 0 && a.dirty[bcell] == 0;
+    }
+    mode = list_ok ? 1 : (ext >= 0 ? 2 : 3);
+  }
+  const unsigned long long fb = __ballot(mode >= 2);
+  if (a.stats && fb && lane == __ffsll((long long)fb) - 1) {
+    // count particles that scanned the full stencil (one atomic per wave)
+    atomicAdd(a.stats, (int)__popcll(fb));
+  }
+  const unsigned long long scan = __ballot(mode == 2);
+  const bool to_queue = a.use_lists && scan && __popcll(scan) <= a.fb_wave_max;
+  if (to_queue) {
+    const int leader = __ffsll((long long)scan) - 1;
+    int at0 = 0;
+    if (lane == leader) at0 = atomicAdd(&queued, (int)__popcll(scan));
+    at0 = __shfl(at0, leader, 64);
+    if (mode == 2) queue[at0 + __popcll(scan & ((1ull << lane) - 1ull))] = i;
   }
   const float rcb = sc.rb + fmaxf(bi.w, sc.rmin);
   const float preb = contact_prefilter(rcb);
   float3 ab = make_float3(0.0f, 0.0f, 0.0f), ag = make_float3(0.0f, 0.0f, 0.0f);
   int nb = 0, ng = 0;
-  if (list_ok) {
+  if (mode == 1) {
     for (int t = 0; t < mt.y; ++t) {
       const int k = a.blist[(size_t)t * sc.cap + i];
       boundary_contact<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, ab, nb);
@@ -258,15 +385,27 @@ __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
       const int j = gl[t * stride];
       granular_contact<FRIC>(bi, oi, a.base[j], a.off_in[j], sc, ag, ng);
     }
-  } else {
-    if (a.stats) {
-      // count particles that scanned the full stencil (one atomic per wave)
-      const unsigned long long fb = __ballot(1);
-      if ((threadIdx.x & 63) == __ffsll((long long)fb) - 1) atomicAdd(a.stats, (int)__popcll(fb));
-    }
-    if (ext >= 0) full_stencil<FRIC>(a, i, bi, oi, bx, by, bz, rcb, preb, ab, ag, nb, ng);
+  } else if (mode == 2 && !to_queue) {
+    full_stencil<FRIC>(a, i, bi, oi, bx, by, bz, rcb, preb, ab, ag, nb, ng);
   }
-  sweep_store<LAST_STAB>(a, i, bi, oi, ab, ag, nb + ng, mt.w);
+  if (mode == 1 || mode == 3 || (mode == 2 && !to_queue))
+    sweep_store<LAST_STAB>(a, i, bi, oi, ab, ag, nb + ng, mt.w);
+  if (!a.use_lists) return;
+  // queued particles: one wave each
+  __syncthreads();
+  const int nq = queued;
+  for (int q = threadIdx.x >> 6; q < nq; q += BS / 64) {
+    const int iq = queue[q];
+    const float4 bq = a.base[iq];
+    const float4 oq = a.off_in[iq];
+    int qx = 0, qy = 0, qz = 0;
+    trunc_cell(bq.x + oq.x, bq.y + oq.y, bq.z + oq.z, g, qx, qy, qz);
+    const float rq = sc.rb + fmaxf(bq.w, sc.rmin);
+    float3 qb = make_float3(0.0f, 0.0f, 0.0f), qg = qb;
+    int nqb = 0, nqg = 0;
+    wave_stencil<FRIC>(a, iq, bq, oq, qx, qy, qz, rq, contact_prefilter(rq), qb, qg, nqb, nqg);
+    if (lane == 0) sweep_store<LAST_STAB>(a, iq, bq, oq, qb, qg, nqb + nqg, a.meta[iq].w);
+  }
 }
 
 struct FinalConst {
@@ -443,7 +582,10 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   a.sc = sweep_const(p, ws);
   a.use_lists = lists_on(p, ws) ? 1 : 0;
   a.remap = (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1;
-  const int bs = block_of(p->sweep_block);
+  a.fb_wave_max = (p->flags & GG_FLAG_FALLBACK_LANE)   ? 0
+                  : (p->flags & GG_FLAG_FALLBACK_WAVE) ? 64
+                                                       : kFallbackWaveMax;
+  const int bs = p->sweep_block == 0 ? 64 : p->sweep_block;
   const int stage = k < S ? GG_STAGE_STAB : GG_STAGE_SWEEP;
   prof_begin(stage, st);
   int rc;

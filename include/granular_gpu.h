@@ -72,13 +72,19 @@ typedef struct gg_params {
   float skin;               /* neighbour-list skin; <= 0 disables lists     */
   int flags;                /* GG_FLAG_*                                    */
   int sweep_block;          /* threads per block of the contact sweeps:
-                               64, 128, 256 or 512; 0 = 256 (autotuned by
+                               64, 128, 256 or 512; 0 = 64 (autotuned by
                                gg_batch_autotune)                           */
-  int nlist_block;          /* same for the neighbour-list build            */
+  int nlist_block;          /* same for the neighbour-list build; 0 = 256   */
 } gg_params;
 
 #define GG_FLAG_NO_NLIST 1     /* always scan the 27-cell stencil */
 #define GG_FLAG_NO_XCD_REMAP 2 /* keep hardware block order (no XCD grouping) */
+/* Particles whose neighbour list is not valid for a sweep scan the 27-cell
+ * stencil instead: one wave per particle when few lanes of a wave need it,
+ * one lane per particle otherwise.  These force one way; results are the
+ * same bit for bit. */
+#define GG_FLAG_FALLBACK_LANE 4
+#define GG_FLAG_FALLBACK_WAVE 8
 
 /* Device arrays of one particle set.  Mirrors GranularParticles
  * (GranularParticles.hpp:5-110) + Particles (Particles.hpp:7-62).
