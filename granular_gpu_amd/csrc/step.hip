@@ -39,6 +39,7 @@ __global__ void __launch_bounds__(kBlock)
                       const float *__restrict__ pos, const float *__restrict__ vel,
                       const float *__restrict__ mass, float4 *__restrict__ x0,
                       float4 *__restrict__ off0, float4 *__restrict__ d0,
+                      float4 *__restrict__ xs,
                       float *__restrict__ mass_tmp, float *__restrict__ ms_out,
                       int *__restrict__ p2c, const int *__restrict__ rem_in,
                       const float *__restrict__ mer_in, const int *__restrict__ mc_in,
@@ -62,6 +63,7 @@ __global__ void __launch_bounds__(kBlock)
   x0[i] = make_float4(px, py, pz, r);
   off0[i] = make_float4(Dx, Dy, Dz, 0.0f);
   d0[i] = make_float4(Dx, Dy, Dz, 0.0f);
+  xs[i] = make_float4(px + Dx, py + Dy, pz + Dz, r);
   mass_tmp[i] = m;
   if (ms_out) ms_out[i] = m * exp_f(-py);
   if (perm && p2c) p2c[i] = key[o];
@@ -71,31 +73,30 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // Candidate lists: particles of the stencil of trunc(x*/L) with
-// d < rc + skin, in scan order (cells ascending, index ascending).
+// |x*_i - x*_j| < r_i + r_j + skin (boundary: |x*_i - q| < R_b + max(r_i,
+// rmin) + skin), in scan order (cells ascending, index ascending).  x* is
+// the predicted position fl(base + offset) with the contact radius in w.
 template <int BS>
 __global__ void __launch_bounds__(BS)
-    k_nlist(int n, const float4 *__restrict__ x, const float4 *__restrict__ off,
+    k_nlist(int n, const float4 *__restrict__ x, const float4 *__restrict__ xs,
             const float4 *__restrict__ bnd4, const int *__restrict__ cs,
             const int *__restrict__ bcs, DevGrid g, SweepConst sc, float skin,
             int4 *__restrict__ meta, int *__restrict__ glist, int *__restrict__ blist,
             int remap) {
   const int i = xcd_block(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 bi = x[i];
-  const float4 oi = off[i];
+  const float4 si = xs[i];
   int bx = 0, by = 0, bz = 0;
-  const bool ok = trunc_cell(bi.x + oi.x, bi.y + oi.y, bi.z + oi.z, g, bx, by, bz);
+  const bool ok = trunc_cell(si.x, si.y, si.z, g, bx, by, bz);
   const int ext = ext_base_id(ok, bx, by, bz, g);
-  const float rbi = fmaxf(bi.w, sc.rmin);
+  const float rbi = fmaxf(si.w, sc.rmin);
   const float blim = (sc.rb + rbi) + skin;
   int gc = 0, bc = 0;
   if (ext >= 0) {
     stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
       for (int k = k0; k < k1; ++k) {
         const float4 q = bnd4[k];
-        const float px = (bi.x - q.x) + oi.x;
-        const float py = (bi.y - q.y) + oi.y;
-        const float pz = (bi.z - q.z) + oi.z;
+        const float px = si.x - q.x, py = si.y - q.y, pz = si.z - q.z;
         const float d2 = px * px + py * py + pz * pz;
         if (d2 < blim * blim) {
           if (bc < sc.bmax) blist[(size_t)bc * sc.cap + i] = k;
@@ -103,12 +104,10 @@ __global__ void __launch_bounds__(BS)
         }
       }
     });
-    auto test = [&](int j, const float4 &bj, const float4 &oj) {
-      const float px = (bi.x - bj.x) + (oi.x - oj.x);
-      const float py = (bi.y - bj.y) + (oi.y - oj.y);
-      const float pz = (bi.z - bj.z) + (oi.z - oj.z);
+    auto test = [&](int j, const float4 &sj) {
+      const float px = si.x - sj.x, py = si.y - sj.y, pz = si.z - sj.z;
       const float d2 = px * px + py * py + pz * pz;
-      const float lim = (bi.w + bj.w) + skin;
+      const float lim = (si.w + sj.w) + skin;
       if (d2 < lim * lim && j != i) {
         if (gc < sc.gmax) glist[(size_t)gc * sc.cap + i] = j;
         ++gc;
@@ -117,16 +116,16 @@ __global__ void __launch_bounds__(BS)
     stencil_rows(bx, by, bz, g, cs, [&](int j0, int j1) {
       int j = j0;
       for (; j + 4 <= j1; j += 4) {
-        const float4 b0 = x[j], b1 = x[j + 1], b2 = x[j + 2], b3 = x[j + 3];
-        const float4 o0 = off[j], o1 = off[j + 1], o2 = off[j + 2], o3 = off[j + 3];
-        test(j, b0, o0);
-        test(j + 1, b1, o1);
-        test(j + 2, b2, o2);
-        test(j + 3, b3, o3);
+        const float4 s0 = xs[j], s1 = xs[j + 1], s2 = xs[j + 2], s3 = xs[j + 3];
+        test(j, s0);
+        test(j + 1, s1);
+        test(j + 2, s2);
+        test(j + 3, s3);
       }
-      for (; j < j1; ++j) test(j, x[j], off[j]);
+      for (; j < j1; ++j) test(j, xs[j]);
     });
   }
+  const float4 bi = x[i];
   meta[i] = make_int4(gc, bc, ext, hash_cell(bi.x, bi.y, bi.z, g));
 }
 
@@ -487,23 +486,23 @@ static SweepConst sweep_const(const gg_params *p, const gg_workspace *ws) {
 }
 
 template <int BS>
-static int launch_nlist_bs(int n, const float4 *x, const float4 *off, const float4 *bnd4,
+static int launch_nlist_bs(int n, const float4 *x, const float4 *xs, const float4 *bnd4,
                            const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
                            int4 *meta, int *glist, int *blist, int remap, hipStream_t st) {
-  k_nlist<BS><<<(n + BS - 1) / BS, BS, 0, st>>>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta,
+  k_nlist<BS><<<(n + BS - 1) / BS, BS, 0, st>>>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta,
                                                  glist, blist, remap);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
 
-static int launch_nlist(int bs, int n, const float4 *x, const float4 *off, const float4 *bnd4,
+static int launch_nlist(int bs, int n, const float4 *x, const float4 *xs, const float4 *bnd4,
                         const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
                         int4 *meta, int *glist, int *blist, int remap, hipStream_t st) {
   switch (bs) {
-    case 64: return launch_nlist_bs<64>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
-    case 128: return launch_nlist_bs<128>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
-    case 512: return launch_nlist_bs<512>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
-    default: return launch_nlist_bs<256>(n, x, off, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+    case 64: return launch_nlist_bs<64>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+    case 128: return launch_nlist_bs<128>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+    case 512: return launch_nlist_bs<512>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+    default: return launch_nlist_bs<256>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
   }
 }
 
@@ -522,7 +521,8 @@ static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, cons
   k_permute_predict<<<nblk, kBlock, 0, st>>>(
       n, perm, key, reinterpret_cast<const float *>(set->pos),
       reinterpret_cast<const float *>(set->vel), set->mass, at<float4>(ws, L.x0),
-      at<float4>(ws, L.off0), at<float4>(ws, L.d0), at<float>(ws, L.ftmp0), set->scaled_mass,
+      at<float4>(ws, L.off0), at<float4>(ws, L.d0), at<float4>(ws, L.off1), at<float>(ws, L.ftmp0),
+      set->scaled_mass,
       set->particle_2_cell, ad ? ad->remove : nullptr, ad ? ad->merge : nullptr,
       ad ? ad->merge_count : nullptr, at<int>(ws, L.itmp0), at<float>(ws, L.ftmp1),
       at<int>(ws, L.itmp1), p->dt, dvx, dvy, dvz, denom, at<int>(ws, L.dirty), ndirty,
@@ -531,7 +531,7 @@ static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, cons
   prof_end(GG_STAGE_PREDICT, st);
   if (lists_on(p, ws) && n > 0) {
     prof_begin(GG_STAGE_NLIST, st);
-    GG_TRY(launch_nlist(block_of(p->nlist_block), n, at<float4>(ws, L.x0), at<float4>(ws, L.off0),
+    GG_TRY(launch_nlist(block_of(p->nlist_block), n, at<float4>(ws, L.x0), at<float4>(ws, L.off1),
                         at<float4>(ws, L.bnd4), cs, bcs, dev_grid(ws->grid), sweep_const(p, ws),
                         p->skin, at<int4>(ws, L.meta), at<int>(ws, L.glist),
                         at<int>(ws, L.blist), (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1, st));

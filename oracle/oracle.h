@@ -88,8 +88,9 @@ int oracle_sweep_f32(int n, const float *base4, const float *off4,
 
 /* Neighbour candidate lists (granular + boundary) in scan order, as the GPU
  * builds them: candidate iff in the stencil of trunc(xstar / L) and
- * |p|^2 < lim^2 with lim = r_i + r_j + skin (granular) or
- * R_b + max(r_i,rmin) + skin (boundary).  base_cell = padded stencil-base id.
+ * |p|^2 < lim^2 with p = xstar_i - xstar_j (xstar = fl(base + off)) or
+ * xstar_i - q, lim = r_i + r_j + skin (granular) or R_b + max(r_i,rmin) +
+ * skin (boundary).  base_cell = padded stencil-base id.
  * Lists are written row-major [i][k]; counts may exceed gmax/bmax (then
  * only the first gmax/bmax entries are stored).                           */
 int oracle_nlist_f32(int n, const float *base4, const float *off4, int nb,

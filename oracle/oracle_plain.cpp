@@ -633,9 +633,9 @@ extern "C" int oracle_nlist_f32(int n, const float *base4, const float *off4,
         if (cell < 0) continue;
         for (int k = bcs[cell]; k < bcs[cell + 1]; ++k) {
           V3<float> p3;
-          p3.x = (bi.x - bp[k].x) + oi.x;
-          p3.y = (bi.y - bp[k].y) + oi.y;
-          p3.z = (bi.z - bp[k].z) + oi.z;
+          p3.x = xs.x - bp[k].x;
+          p3.y = xs.y - bp[k].y;
+          p3.z = xs.z - bp[k].z;
           const float d2 = p3.x * p3.x + p3.y * p3.y + p3.z * p3.z;
           const float lim = (sp.rb + rbi) + skin;
           if (d2 < lim * lim) {
@@ -645,10 +645,13 @@ extern "C" int oracle_nlist_f32(int n, const float *base4, const float *off4,
         }
         for (int j = cs[cell]; j < cs[cell + 1]; ++j) {
           if (j == i) continue;
+          const V3<float> bj{base4[4 * j], base4[4 * j + 1], base4[4 * j + 2]};
+          const V3<float> oj{off4[4 * j], off4[4 * j + 1], off4[4 * j + 2]};
+          const V3<float> xsj = add(bj, oj);
           V3<float> p3;
-          p3.x = (bi.x - base4[4 * j]) + (oi.x - off4[4 * j]);
-          p3.y = (bi.y - base4[4 * j + 1]) + (oi.y - off4[4 * j + 1]);
-          p3.z = (bi.z - base4[4 * j + 2]) + (oi.z - off4[4 * j + 2]);
+          p3.x = xs.x - xsj.x;
+          p3.y = xs.y - xsj.y;
+          p3.z = xs.z - xsj.z;
           const float d2 = p3.x * p3.x + p3.y * p3.y + p3.z * p3.z;
           const float lim = (ri + base4[4 * j + 3]) + skin;
           if (d2 < lim * lim) {
