@@ -39,6 +39,8 @@ struct Layout {
   size_t ctr;      // int[64]        device counters ([0] = big-cell count)
   size_t ctr2;     // int[64]        adaptive counters
   size_t dirty;    // int[C+1]       cells near a particle past half the skin
+  size_t bounds;   // uint[4]        largest |D| component and contact radius of the step (float bits)
+  size_t bpart;    // uint[2*blocks] per-block maxima reduced into bounds
   size_t x0;       // float4[cap]    start-of-step position + radius
   size_t off0;     // float4[cap]    offset ping
   size_t off1;     // float4[cap]    offset pong
@@ -101,6 +103,8 @@ inline Layout make_layout(int capacity, int bcap, gg_grid g, int max_sweeps,
   L.ctr = take(64 * 4);
   L.ctr2 = take(64 * 4);
   L.dirty = take(C1 * 4);
+  L.bounds = take(16);
+  L.bpart = take(nblk0 * 8);
   L.x0 = take(cap * 16);
   L.off0 = take(cap * 16);
   L.off1 = take(cap * 16);

@@ -46,30 +46,83 @@ __global__ void __launch_bounds__(kBlock)
                       int *__restrict__ rem_tmp, float *__restrict__ mer_tmp,
                       int *__restrict__ mc_tmp, float dt, float dvx, float dvy, float dvz,
                       double denom, int *__restrict__ dirty, int ndirty,
-                      int *__restrict__ fbcount) {
+                      int *__restrict__ fbcount, unsigned int *__restrict__ bpart) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int k = i; k < ndirty; k += gridDim.x * blockDim.x) dirty[k] = 0;
   if (i < kFallbackSlots) fbcount[i] = 0;
-  if (i >= n) return;
-  const int o = perm ? perm[i] : i;
-  const float px = pos[3 * o], py = pos[3 * o + 1], pz = pos[3 * o + 2];
-  float vx = vel[3 * o], vy = vel[3 * o + 1], vz = vel[3 * o + 2];
-  vx = vx + dvx;
-  vy = vy + dvy;
-  vz = vz + dvz;
-  const float Dx = dt * vx, Dy = dt * vy, Dz = dt * vz;
-  const float m = mass[o];
-  const float r = radius_contact(m, denom);
-  x0[i] = make_float4(px, py, pz, r);
-  off0[i] = make_float4(Dx, Dy, Dz, 0.0f);
-  d0[i] = make_float4(Dx, Dy, Dz, 0.0f);
-  xs[i] = make_float4(px + Dx, py + Dy, pz + Dz, r);
-  mass_tmp[i] = m;
-  if (ms_out) ms_out[i] = m * exp_f(-py);
-  if (perm && p2c) p2c[i] = key[o];
-  if (rem_in) rem_tmp[i] = rem_in[o];
-  if (mer_in) mer_tmp[i] = mer_in[o];
-  if (mc_in) mc_tmp[i] = mc_in[o];
+  float dmax = 0.0f, rmax = 0.0f;
+  if (i < n) {
+    const int o = perm ? perm[i] : i;
+    const float px = pos[3 * o], py = pos[3 * o + 1], pz = pos[3 * o + 2];
+    float vx = vel[3 * o], vy = vel[3 * o + 1], vz = vel[3 * o + 2];
+    vx = vx + dvx;
+    vy = vy + dvy;
+    vz = vz + dvz;
+    const float Dx = dt * vx, Dy = dt * vy, Dz = dt * vz;
+    const float m = mass[o];
+    const float r = radius_contact(m, denom);
+    x0[i] = make_float4(px, py, pz, r);
+    off0[i] = make_float4(Dx, Dy, Dz, 0.0f);
+    d0[i] = make_float4(Dx, Dy, Dz, 0.0f);
+    xs[i] = make_float4(px + Dx, py + Dy, pz + Dz, r);
+    mass_tmp[i] = m;
+    if (ms_out) ms_out[i] = m * exp_f(-py);
+    if (perm && p2c) p2c[i] = key[o];
+    if (rem_in) rem_tmp[i] = rem_in[o];
+    if (mer_in) mer_tmp[i] = mer_in[o];
+    if (mc_in) mc_tmp[i] = mc_in[o];
+    dmax = fmaxf(fabsf(Dx), fmaxf(fabsf(Dy), fabsf(Dz)));
+    if (!(Dx == Dx && Dy == Dy && Dz == Dz)) dmax = __builtin_inff();
+    rmax = r == r ? r : __builtin_inff();
+  }
+  // per-block largest |D| component and radius; k_reduce_bounds folds the
+  // blocks
+  __shared__ float red[2][kBlock / 64];
+  const float wd = wave_max(dmax);
+  const float wr = wave_max(rmax);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = wd;
+    red[1][threadIdx.x >> 6] = wr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float a = red[0][0], b = red[1][0];
+    for (int w = 1; w < kBlock / 64; ++w) {
+      a = fmaxf(a, red[0][w]);
+      b = fmaxf(b, red[1][w]);
+    }
+    bpart[2 * blockIdx.x] = __float_as_uint(a);
+    bpart[2 * blockIdx.x + 1] = __float_as_uint(b);
+  }
+}
+
+// bounds[0..1] = maxima of the nb per-block values (non-negative floats
+// order like their bit patterns).  One block.
+__global__ void __launch_bounds__(1024) k_reduce_bounds(const unsigned int *__restrict__ bpart,
+                                                        int nb, unsigned int *__restrict__ bounds) {
+  __shared__ unsigned int red[2][16];
+  unsigned int a = 0u, b = 0u;
+  for (int k = threadIdx.x; k < nb; k += 1024) {
+    a = max(a, bpart[2 * k]);
+    b = max(b, bpart[2 * k + 1]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    a = max(a, (unsigned int)__shfl_xor((int)a, o, 64));
+    b = max(b, (unsigned int)__shfl_xor((int)b, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < 16; ++w) {
+      a = max(a, red[0][w]);
+      b = max(b, red[1][w]);
+    }
+    bounds[0] = a;
+    bounds[1] = b;
+  }
 }
 
 // Candidate lists: particles of the stencil of trunc(x*/L) with
@@ -149,6 +202,8 @@ struct SweepArgs {
   int use_lists;
   int remap;
   int fb_wave_max;  // a wave hands at most this many fallbacks to whole waves
+  const unsigned int *__restrict__ bounds;  // largest |D| component, contact radius
+  float near_limit;  // a list stays usable one cell over while its reach is below this
 };
 
 // Shared epilogue: new offset (and for the last stabilisation sweep the
@@ -338,7 +393,18 @@ __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
     if (a.use_lists) {
       mt = a.meta[i];
       const int bcell = base_cell_id(ok, bx, by, bz, g);
-      list_ok = ext == mt.z && mt.x <= sc.gmax && mt.y <= sc.bmax && oi.w < sc.half_valid &&
+      bool base_ok = ext == mt.z;
+      if (!base_ok && ext >= 0) {
+        // The stencil base moved one cell (x* moved less than half the
+        // skin).  Scan order is ascending index for every base, and no pair
+        // entering or leaving the stencil can be in contact while
+        // r_i + r_max + |D|_max + 2 * half_valid < L, so the list still
+        // holds exactly the contacts of the new stencil.
+        const float dmax = __uint_as_float(a.bounds[0]), rmax = __uint_as_float(a.bounds[1]);
+        const float reach = fmaxf((bi.w + rmax) + dmax, sc.rb + fmaxf(bi.w, sc.rmin));
+        base_ok = reach + 2.0f * sc.half_valid < a.near_limit;
+      }
+      list_ok = base_ok && mt.x <= sc.gmax && mt.y <= sc.bmax && oi.w < sc.half_valid &&
                 bcell >= 0 && a.dirty[bcell] == 0;
     }
     mode = list_ok ? 1 : (ext >= 0 ? 2 : 3);
@@ -526,7 +592,10 @@ static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, cons
       set->particle_2_cell, ad ? ad->remove : nullptr, ad ? ad->merge : nullptr,
       ad ? ad->merge_count : nullptr, at<int>(ws, L.itmp0), at<float>(ws, L.ftmp1),
       at<int>(ws, L.itmp1), p->dt, dvx, dvy, dvz, denom, at<int>(ws, L.dirty), ndirty,
-      at<int>(ws, L.ctr) + 1);
+      at<int>(ws, L.ctr) + 1, at<unsigned int>(ws, L.bpart));
+  GG_LAUNCH_CHECK();
+  k_reduce_bounds<<<1, 1024, 0, st>>>(at<unsigned int>(ws, L.bpart), nblk,
+                                      at<unsigned int>(ws, L.bounds));
   GG_LAUNCH_CHECK();
   prof_end(GG_STAGE_PREDICT, st);
   if (lists_on(p, ws) && n > 0) {
@@ -582,6 +651,8 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   a.sc = sweep_const(p, ws);
   a.use_lists = lists_on(p, ws) ? 1 : 0;
   a.remap = (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1;
+  a.bounds = at<unsigned int>(ws, L.bounds);
+  a.near_limit = 0.999f * ws->grid.cell_length;
   a.fb_wave_max = (p->flags & GG_FLAG_FALLBACK_LANE)   ? 0
                   : (p->flags & GG_FLAG_FALLBACK_WAVE) ? 64
                                                        : kFallbackWaveMax;
