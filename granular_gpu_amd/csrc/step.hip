@@ -207,8 +207,10 @@ struct SweepArgs {
 };
 
 // Shared epilogue: new offset (and for the last stabilisation sweep the
-// position base), dirty flags when the particle first passes half the
-// valid skin.
+// position base).  A particle whose accumulated displacement has reached
+// half the valid skin raises the displacement mark of the 27 cells around
+// its sorted cell to that displacement (every sweep while it moves), so that
+// particles scanning those cells can tell whether their lists still hold.
 template <bool LAST_STAB>
 __device__ __forceinline__ void sweep_store(const SweepArgs &a, int i, const float4 &bi,
                                             const float4 &oi, float3 ab, float3 ag, int cnt,
@@ -219,15 +221,15 @@ __device__ __forceinline__ void sweep_store(const SweepArgs &a, int i, const flo
   const float iz = (ab.z + ag.z) / nf;
   const float ox = oi.x + ix, oy = oi.y + iy, oz = oi.z + iz;
   const float cum_new = oi.w + sqrtf(ix * ix + iy * iy + iz * iz);
-  if (a.use_lists && oi.w < a.sc.half_valid && !(cum_new < a.sc.half_valid)) {
-    // first sweep past half the valid skin: flag the stencil around the
-    // particle's cell so that neighbours stop trusting their lists.
+  if (a.use_lists && !(cum_new < a.sc.half_valid) && cum_new != oi.w) {
     const DevGrid &g = a.g;
     if (cell >= 0 && cell < g.cx * g.cy * g.cz) {
       const int cz0 = cell % g.cz, cy0 = (cell / g.cz) % g.cy, cx0 = cell / (g.cz * g.cy);
+      // non-negative floats (and NaN above them) order like their bits
+      const unsigned int mark = __float_as_uint(cum_new);
       for (int m = 0; m < 27; ++m) {
         const int cc = stencil_cell(cx0, cy0, cz0, m, g);
-        if (cc >= 0) a.dirty[cc] = 1;
+        if (cc >= 0) atomicMax(reinterpret_cast<unsigned int *>(a.dirty) + cc, mark);
       }
     }
   }
@@ -404,8 +406,15 @@ __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
         const float reach = fmaxf((bi.w + rmax) + dmax, sc.rb + fmaxf(bi.w, sc.rmin));
         base_ok = reach + 2.0f * sc.half_valid < a.near_limit;
       }
-      list_ok = base_ok && mt.x <= sc.gmax && mt.y <= sc.bmax && oi.w < sc.half_valid &&
-                bcell >= 0 && a.dirty[bcell] == 0;
+      // A pair outside the list stays out of contact while
+      // cum_i + cum_j < 2 * half_valid (0.9 skin); cum_j < half_valid
+      // unless j marked the cell.
+      bool moved_ok = false;
+      if (bcell >= 0 && oi.w < sc.half_valid) {
+        const unsigned int mark = reinterpret_cast<const unsigned int *>(a.dirty)[bcell];
+        moved_ok = mark == 0u || oi.w + __uint_as_float(mark) < 2.0f * sc.half_valid;
+      }
+      list_ok = base_ok && moved_ok && mt.x <= sc.gmax && mt.y <= sc.bmax;
     }
     mode = list_ok ? 1 : (ext >= 0 ? 2 : 3);
   }
