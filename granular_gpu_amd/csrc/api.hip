@@ -58,7 +58,7 @@ extern "C" void gg_params_default(gg_params *p) {
   p->damp_speed = 1.0f;           // Solver.cu:202
   p->damping = 0.99f;             // Solver.cu:204-205
   p->adaptive_damping = 0.98f;    // Solver.cu:242
-  p->skin = 0.005f;
+  p->skin = 0.0075f;
   p->flags = 0;
 }
 

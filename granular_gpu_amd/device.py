@@ -27,7 +27,7 @@ def _stream():
 class PlainSim:
     """Plain (non-adaptive) scene on one GPU."""
 
-    def __init__(self, scene, stab=3, solver=5, mu_s=0.0, mu_k=0.0, skin=0.005, gmax=48,
+    def __init__(self, scene, stab=3, solver=5, mu_s=0.0, mu_k=0.0, skin=0.0075, gmax=48,
                  bmax=16, flags=0, capacity=None, device="cuda", max_sweeps=None):
         lib = load()
         self.lib = lib

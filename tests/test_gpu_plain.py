@@ -83,7 +83,7 @@ def test_step_bitexact_vs_fp32_oracle(gpu, steps, stab, solver, fric, lists):
     _log(f"step steps={steps} S={stab} I={solver} fric={fric} lists={lists}")
     s = scenes.plain_box(32, 1.0)  # 32,768 particles, 25^3 cells, 14,408 walls
     sim = _sim(s, stab=stab, solver=solver, mu_s=fric[0], mu_k=fric[1],
-               skin=0.005 if lists else 0.0)
+               skin=0.0075 if lists else 0.0)
     sim.step(steps)
     gp, gv, gm = sim.host()
     op, ov, om, oms, op2c, _ = _oracle_steps(s, steps, stab, solver, *fric)
@@ -113,7 +113,7 @@ def test_neighbour_lists_bitexact(gpu):
     _log("neighbour lists")
     s = scenes.plain_box(24, 1.0)
     gmax, bmax = 48, 16
-    sim = _sim(s, gmax=gmax, bmax=bmax)
+    sim = _sim(s, gmax=gmax, bmax=bmax, skin=0.0075)
     sim.grid_build()
     sim.predict()
     torch.cuda.synchronize()
@@ -126,7 +126,7 @@ def test_neighbour_lists_bitexact(gpu):
     bl = sim.view(4, bmax * cap, dtype=torch.int32, width=1).reshape(bmax, cap)[:, :n]
     x0, off, meta = x0.cpu().numpy(), off.cpu().numpy(), meta.cpu().numpy()
     gl, bl = gl.cpu().numpy(), bl.cpu().numpy()
-    p = po.params_from(s, skin=0.005)
+    p = po.params_from(s, skin=0.0075)
     bpos, bcs = po.sorted_boundary(s.boundary, s.cell_length, s.cells)
     cs = sim.cs.cpu().numpy()
     ogc = np.empty(n, np.int32); ogl = np.empty(n * gmax, np.int32)
