@@ -157,6 +157,7 @@ _PROTOS = {
     "gg_compact": (C.c_int, [VP, VP, C.c_longlong, C.c_int, VP, VP, VP, C.c_size_t, VP]),
     "gg_fill_u32": (C.c_int, [VP, C.c_size_t, C.c_uint32, VP]),
     "gg_profiler_enable": (C.c_int, [C.c_int]),
+    "gg_profiler_pause": (C.c_int, [C.c_int]),
     "gg_profiler_collect": (C.c_int, [P(C.c_double), P(C.c_int), C.c_int]),
 }
 

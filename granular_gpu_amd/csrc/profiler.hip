@@ -11,6 +11,7 @@ namespace gg {
 namespace {
 struct Prof {
   bool on = false;
+  bool paused = false;
   std::vector<hipEvent_t> ev;
   std::vector<std::pair<int, int>> rec;  // (stage, first event index)
   int next = 0;
@@ -31,7 +32,7 @@ static bool capturing(hipStream_t st) {
 
 void prof_begin(int stage, hipStream_t st) {
   Prof &p = prof();
-  if (!p.on || p.next + 2 > (int)p.ev.size() || capturing(st)) {
+  if (!p.on || p.paused || p.next + 2 > (int)p.ev.size() || capturing(st)) {
     p.open_stage = -1;
     return;
   }
@@ -61,11 +62,17 @@ extern "C" int gg_profiler_enable(int max_records) {
   p.rec.clear();
   p.next = 0;
   p.on = false;
+  p.paused = false;
   if (max_records <= 0) return GG_OK;
   p.ev.resize(2 * (size_t)max_records);
   for (auto &e : p.ev)
     if (hipEventCreate(&e) != hipSuccess) return GG_ERR_HIP;
   p.on = true;
+  return GG_OK;
+}
+
+extern "C" int gg_profiler_pause(int paused) {
+  prof().paused = paused != 0;
   return GG_OK;
 }
 

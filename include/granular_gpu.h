@@ -279,6 +279,9 @@ enum {
   GG_STAGE_COUNT = 12
 };
 int gg_profiler_enable(int max_records); /* 0 disables */
+/* Nonzero stops recording new stages (records so far are kept), 0 resumes;
+ * lets a caller time a sample of the steps of a longer run. */
+int gg_profiler_pause(int paused);
 int gg_profiler_collect(double *total_ms, int *count, int nstages);
 
 /* Read-only views into the workspace for checking (device pointers):
