@@ -41,6 +41,18 @@ __device__ __forceinline__ uint16_t to_bf16(float x) {
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2v __attribute__((ext_vector_type(2)));
+
+// Two floats rounded to bfloat16 by the hardware (v_cvt_pk_bf16_f32), lo in
+// the low half.  Equal to to_bf16 for every non-NaN input:
+// gg_bf16_round_check compares the two over all 2^32 bit patterns.
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  const f32x2v v = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2v));
+}
+
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
@@ -360,6 +372,30 @@ __global__ void k_to_bf16(long long count, const float *__restrict__ x, uint16_t
 // live in LDS, shared by the waves.  A wave takes 16 TILES samples per
 // iteration: TILES MFMA column tiles that share every weight fragment.
 
+// Weights in LDS are stored in fragment order: the A fragment of output
+// tile t and k-step ks for lane l is the 4 bfloat16 at
+// ((t * KSTEPS + ks) * 64 + l) * 4, so a wave's fragment read is one
+// contiguous 512-byte block (no bank conflicts).  Row-major W[o][k] maps to
+// t = o / 16, ks = k / 16, l = o % 16 + 16 ((k % 16) / 4), e = k % 4.
+__device__ __forceinline__ const bf16x4 &frag(const uint16_t *__restrict__ W, int ksteps, int t,
+                                              int ks, int lane) {
+  return *reinterpret_cast<const bf16x4 *>(W + ((t * ksteps + ks) * 64 + lane) * 4);
+}
+
+// Copies one layer's row-major weights (rows x 16*ksteps) into fragment order.
+__device__ __forceinline__ void stage_layer(uint16_t *__restrict__ dst,
+                                            const uint16_t *__restrict__ src, int rows,
+                                            int ksteps) {
+  const int frags = (rows / 16) * ksteps * 64;  // 4-element fragments
+  for (int f = threadIdx.x; f < frags; f += blockDim.x) {
+    const int lane = f & 63, fk = f >> 6;
+    const int t = fk / ksteps, ks = fk % ksteps;
+    const int o = 16 * t + (lane & 15), k = 16 * ks + 4 * (lane >> 4);
+    reinterpret_cast<uint2 *>(dst)[f] =
+        *reinterpret_cast<const uint2 *>(src + (size_t)o * (16 * ksteps) + k);
+  }
+}
+
 // One layer, transposed: output tile t (features 16t..16t+15) of every
 // sample tile from KSTEPS input k-steps; bias, ReLU and bfloat16 packing
 // give the next layer's B operand directly.
@@ -368,7 +404,7 @@ __device__ __forceinline__ void hidden_layer(const uint16_t *__restrict__ W,
                                              const float *__restrict__ bias,
                                              const bf16x4 (&in)[TILES][8],
                                              bf16x4 (&out)[TILES][8], int lane) {
-  const int c = lane & 15, q = lane >> 4;
+  const int q = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
     f32x4 a[TILES];
@@ -376,19 +412,19 @@ __device__ __forceinline__ void hidden_layer(const uint16_t *__restrict__ W,
     for (int s = 0; s < TILES; ++s) a[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ++ks) {
-      const bf16x4 w = *reinterpret_cast<const bf16x4 *>(W + (16 * t + c) * (16 * KSTEPS) +
-                                                         16 * ks + 4 * q);
+      const bf16x4 w = frag(W, KSTEPS, t, ks, lane);
 #pragma unroll
       for (int s = 0; s < TILES; ++s) a[s] = mfma16(w, in[s][ks], a[s]);
     }
+    // bias + ReLU (no NaN survives it), then hardware round-to-nearest-even
+    const float b0 = bias[16 * t + 4 * q], b1 = bias[16 * t + 4 * q + 1];
+    const float b2 = bias[16 * t + 4 * q + 2], b3 = bias[16 * t + 4 * q + 3];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float b = bias[16 * t + 4 * q + e];
-#pragma unroll
-      for (int s = 0; s < TILES; ++s) {
-        const float z = a[s][e] + b;
-        out[s][t][e] = (short)to_bf16(z > 0.0f ? z : 0.0f);
-      }
+    for (int s = 0; s < TILES; ++s) {
+      const float z0 = a[s][0] + b0, z1 = a[s][1] + b1, z2 = a[s][2] + b2, z3 = a[s][3] + b3;
+      const u32x2v h = {pack_bf16x2(z0 > 0.0f ? z0 : 0.0f, z1 > 0.0f ? z1 : 0.0f),
+                        pack_bf16x2(z2 > 0.0f ? z2 : 0.0f, z3 > 0.0f ? z3 : 0.0f)};
+      out[s][t] = __builtin_bit_cast(bf16x4, h);
     }
     // keep each tile's weight loads next to its MFMAs (bounds live registers)
     __builtin_amdgcn_sched_barrier(0);
@@ -401,7 +437,7 @@ __device__ __forceinline__ void output_layer(const uint16_t *__restrict__ W,
                                              const float *__restrict__ bias,
                                              const bf16x4 (&in)[TILES][8],
                                              f32x4 (&y)[TILES][2], int lane) {
-  const int c = lane & 15, q = lane >> 4;
+  const int q = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     f32x4 a[TILES];
@@ -409,7 +445,7 @@ __device__ __forceinline__ void output_layer(const uint16_t *__restrict__ W,
     for (int s = 0; s < TILES; ++s) a[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int ks = 0; ks < 8; ++ks) {
-      const bf16x4 w = *reinterpret_cast<const bf16x4 *>(W + (16 * t + c) * 128 + 16 * ks + 4 * q);
+      const bf16x4 w = frag(W, 8, t, ks, lane);
 #pragma unroll
       for (int s = 0; s < TILES; ++s) a[s] = mfma16(w, in[s][ks], a[s]);
     }
@@ -429,10 +465,12 @@ __global__ void __launch_bounds__(64 * WAVES)
                 const float *__restrict__ pos, const float *__restrict__ vel,
                 const uint16_t *__restrict__ wts, const float *__restrict__ bias_g, float radius,
                 float dt, float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
-  __shared__ uint16_t W[49152];
+  __shared__ __attribute__((aligned(16))) uint16_t W[49152];
   __shared__ float bias[416];
-  for (int i = threadIdx.x; i < 49152 / 2; i += blockDim.x)
-    reinterpret_cast<uint32_t *>(W)[i] = reinterpret_cast<const uint32_t *>(wts)[i];
+  stage_layer(W + kWOff1, wts + kWOff1, kHid, kIn / 16);
+  stage_layer(W + kWOff2, wts + kWOff2, kHid, kHid / 16);
+  stage_layer(W + kWOff3, wts + kWOff3, kHid, kHid / 16);
+  stage_layer(W + kWOff4, wts + kWOff4, kOutPad, kHid / 16);
   for (int i = threadIdx.x; i < 416; i += blockDim.x) bias[i] = bias_g[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -790,6 +828,33 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   GG_LAUNCH_CHECK();
   prof_end(GG_STAGE_MLP_FALLBACK, stream);
   (void)fallback_count;
+  return GG_OK;
+}
+
+// Counts the non-NaN float bit patterns first .. first + count - 1 (mod
+// 2^32) on which pack_bf16x2 and to_bf16 disagree (either half).
+__global__ void __launch_bounds__(256)
+    k_bf16_round_check(uint64_t first, uint64_t count, unsigned long long *__restrict__ bad) {
+  unsigned long long local = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+    const uint32_t u = (uint32_t)(first + k);
+    if ((u & 0x7fffffffu) > 0x7f800000u) continue;
+    const float x = __uint_as_float(u);
+    const float y = __uint_as_float(u ^ 0x80000000u);
+    const uint32_t hw = pack_bf16x2(x, y);
+    if ((hw & 0xffffu) != to_bf16(x) || (hw >> 16) != to_bf16(y)) ++local;
+  }
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd(bad, local);
+}
+
+extern "C" int gg_bf16_round_check(uint64_t first, uint64_t count, unsigned long long *mismatches,
+                                   hipStream_t stream) {
+  if (!mismatches) return GG_ERR_INVALID;
+  if (count == 0) return GG_OK;
+  k_bf16_round_check<<<2048, 256, 0, stream>>>(first, count, mismatches);
+  GG_LAUNCH_CHECK();
   return GG_OK;
 }
 

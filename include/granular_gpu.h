@@ -370,6 +370,12 @@ int gg_upsample_mlp(const gg_particles *coarse, const int *cell_start,
                     size_t scratch_bytes, int *fallback_count, hipStream_t stream);
 /* One 16x16x16 bfloat16 MFMA, D = A B (row-major): checks the lane layout. */
 int gg_mfma_probe(const uint16_t *a, const uint16_t *b, float *d, hipStream_t stream);
+/* Adds to *mismatches (device counter) the number of non-NaN float bit
+ * patterns first .. first+count-1 (mod 2^32) whose hardware bfloat16 rounding
+ * (used by the inference kernel's hidden layers) differs from the
+ * round-to-nearest-even the header defines. */
+int gg_bf16_round_check(uint64_t first, uint64_t count, unsigned long long *mismatches,
+                        hipStream_t stream);
 
 /* ---- batched scenes -------------------------------------------------------- */
 /* Many independent plain scenes advanced together by one hipGraph launch

@@ -59,6 +59,18 @@ def test_mfma_lane_layout(gpu):
     assert np.array_equal(d.cpu().numpy(), A @ B)
 
 
+def test_bf16_hardware_rounding_matches_definition(gpu):
+    # the inference kernel rounds hidden activations with v_cvt_pk_bf16_f32;
+    # it must equal the defined round-to-nearest-even on every non-NaN float
+    from granular_gpu_amd import _lib
+    from granular_gpu_amd.device import _ptr, _stream
+    bad = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _lib.check("gg_bf16_round_check",
+               _lib.load().gg_bf16_round_check(0, 1 << 32, _ptr(bad), _stream()))
+    torch.cuda.synchronize()
+    assert int(bad.item()) == 0
+
+
 def test_features_and_teacher_bitexact(gpu):
     from granular_gpu_amd.learned import LearnedUpsampler, MlpNet
     s, sim = _pile()
