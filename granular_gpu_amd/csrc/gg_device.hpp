@@ -97,6 +97,15 @@ __device__ __forceinline__ void stencil_rows(int bx, int by, int bz, const DevGr
   }
 }
 
+// Element i of a packed float3 array (x, y, z at 3i), read with one 12-byte
+// load per lane instead of three scalar ones.
+struct Packed3 {
+  float x, y, z;
+};
+__device__ __forceinline__ Packed3 ld3(const float *__restrict__ p, long long i) {
+  return reinterpret_cast<const Packed3 *>(p)[i];
+}
+
 // exp / cbrt evaluated in double and rounded once to float.
 __device__ __forceinline__ float exp_f(float x) { return (float)exp((double)x); }
 __device__ __forceinline__ float cbrt_f(float x) { return (float)cbrt((double)x); }

@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "gg_device.hpp"
 #include "gg_internal.hpp"
@@ -66,8 +67,16 @@ struct MlpConst {
 };
 
 // ---- features ---------------------------------------------------------------
-// One thread per coarse particle: the 16 smallest (d2, j) of the stencil,
-// kept sorted in registers by an unrolled insertion.
+// Stencil cells nearest-first (centre, 6 faces, 12 edges, 8 corners) as
+// stencil_cell indices m = 9 (dx+1) + 3 (dy+1) + (dz+1).  The 16 smallest
+// keys are a set, so the visiting order does not change them; visiting near
+// cells first means most later candidates are rejected by one comparison.
+__constant__ int kNearFirst[27] = {13, 4,  10, 12, 14, 16, 22, 1,  3,  5,  7,  9,  11, 15,
+                                   17, 19, 21, 23, 25, 0,  2,  6,  8,  18, 20, 24, 26};
+
+// One thread per coarse particle: the 16 smallest (d2, j) of the stencil as
+// 64-bit keys (d2 bits << 32 | j; d2 >= 0 orders like its bits), kept sorted
+// in registers by a branchless insertion.
 // GROUPED = 0: feat[i][96] (sample-major, the training layout);
 // GROUPED = 1: feat[g][i][4] for the 24 groups g of 4 consecutive features
 // (the inference layout: a wave's stores and the network's B-operand loads
@@ -79,55 +88,49 @@ __global__ void __launch_bounds__(256)
                    const int *__restrict__ bcs, DevGrid g, MlpConst c,
                    uint16_t *__restrict__ feat, int *__restrict__ flags, int *__restrict__ knn,
                    int *__restrict__ fb_list, int *__restrict__ fb_count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // consecutive blocks stay on one XCD (spatial neighbours share its L2)
+  const int i = xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float xx = pos[3 * i], xy = pos[3 * i + 1], xz = pos[3 * i + 2];
-  float kd[kKnn];
-  int kj[kKnn];
+  const Packed3 xi = ld3(pos, i);
+  const float xx = xi.x, xy = xi.y, xz = xi.z;
+  constexpr unsigned long long kNone = ~0ull;
+  unsigned long long key[kKnn];
 #pragma unroll
-  for (int s = 0; s < kKnn; ++s) {
-    kd[s] = __builtin_inff();
-    kj[s] = -1;
-  }
+  for (int s = 0; s < kKnn; ++s) key[s] = kNone;
   int flag = 0;
   const float gb2 = c.guard_boundary * c.guard_boundary;
   int bx, by, bz;
   if (floor_cell(xx, xy, xz, g, bx, by, bz)) {
-    stencil_rows(bx, by, bz, g, cs, [&](int j0, int j1) {
-      for (int j = j0; j < j1; ++j) {
-        if (j == i) continue;
-        const float dx = pos[3 * j] - xx, dy = pos[3 * j + 1] - xy, dz = pos[3 * j + 2] - xz;
+    for (int m = 0; m < 27; ++m) {
+      const int cell = stencil_cell(bx, by, bz, kNearFirst[m], g);
+      if (cell < 0) continue;
+      const int j1 = cs[cell + 1];
+      for (int j = cs[cell]; j < j1; ++j) {
+        const Packed3 pj = ld3(pos, j);
+        const float dx = pj.x - xx, dy = pj.y - xy, dz = pj.z - xz;
         const float d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < kd[kKnn - 1] || (d2 == kd[kKnn - 1] && j < kj[kKnn - 1]))) continue;
-        // shift larger entries up by one, place (d2, j)
-        bool placed = false;
+        const unsigned long long k = ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned)j;
+        // finite d2 only (NaN and inf never enter), and not the particle itself
+        if (!(d2 < __builtin_inff()) || j == i || !(k < key[kKnn - 1])) continue;
+        bool lt[kKnn];
 #pragma unroll
-        for (int s = kKnn - 1; s > 0; --s) {
-          const bool larger = d2 < kd[s - 1] || (d2 == kd[s - 1] && j < kj[s - 1]);
-          if (!placed) {
-            if (larger) {
-              kd[s] = kd[s - 1];
-              kj[s] = kj[s - 1];
-            } else {
-              kd[s] = d2;
-              kj[s] = j;
-              placed = true;
-            }
-          }
-        }
-        if (!placed) {
-          kd[0] = d2;
-          kj[0] = j;
-        }
+        for (int s = 0; s < kKnn; ++s) lt[s] = k < key[s];
+#pragma unroll
+        for (int s = kKnn - 1; s > 0; --s) key[s] = lt[s - 1] ? key[s - 1] : (lt[s] ? k : key[s]);
+        key[0] = lt[0] ? k : key[0];
       }
-    });
+    }
     stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
       for (int k = k0; k < k1; ++k) {
-        const float dx = bpos[3 * k] - xx, dy = bpos[3 * k + 1] - xy, dz = bpos[3 * k + 2] - xz;
+        const Packed3 q = ld3(bpos, k);
+        const float dx = q.x - xx, dy = q.y - xy, dz = q.z - xz;
         if (dx * dx + dy * dy + dz * dz < gb2) flag |= 1;
       }
     });
   }
+  int kj[kKnn];
+#pragma unroll
+  for (int s = 0; s < kKnn; ++s) kj[s] = key[s] == kNone ? -1 : (int)(unsigned)(key[s] & 0xffffffffu);
   if (xy < c.guard_floor || xx < c.guard_side || xz < c.guard_side) flag |= 2;
   flags[i] = flag;
   if (flag && fb_list) {
@@ -141,7 +144,8 @@ __global__ void __launch_bounds__(256)
     slot = __shfl(slot, leader, 64) + __popcll(m & ((1ull << lane) - 1ull));
     fb_list[slot] = i;
   }
-  const float vx = vel[3 * i], vy = vel[3 * i + 1], vz = vel[3 * i + 2];
+  const Packed3 vi = ld3(vel, i);
+  const float vx = vi.x, vy = vi.y, vz = vi.z;
   const float inv_r = 1.0f / c.radius, inv_v = 1.0f / c.vscale;
   // two neighbour slots = 12 features = 3 groups of 4
 #pragma unroll
@@ -152,12 +156,13 @@ __global__ void __launch_bounds__(256)
       const int j = kj[s + u];
       if (knn) knn[(size_t)i * kKnn + s + u] = j;
       if (j >= 0) {
-        h[6 * u + 0] = to_bf16((pos[3 * j] - xx) * inv_r);
-        h[6 * u + 1] = to_bf16((pos[3 * j + 1] - xy) * inv_r);
-        h[6 * u + 2] = to_bf16((pos[3 * j + 2] - xz) * inv_r);
-        h[6 * u + 3] = to_bf16((vel[3 * j] - vx) * inv_v);
-        h[6 * u + 4] = to_bf16((vel[3 * j + 1] - vy) * inv_v);
-        h[6 * u + 5] = to_bf16((vel[3 * j + 2] - vz) * inv_v);
+        const Packed3 pj = ld3(pos, j), vj = ld3(vel, j);
+        h[6 * u + 0] = to_bf16((pj.x - xx) * inv_r);
+        h[6 * u + 1] = to_bf16((pj.y - xy) * inv_r);
+        h[6 * u + 2] = to_bf16((pj.z - xz) * inv_r);
+        h[6 * u + 3] = to_bf16((vj.x - vx) * inv_v);
+        h[6 * u + 4] = to_bf16((vj.y - vy) * inv_v);
+        h[6 * u + 5] = to_bf16((vj.z - vz) * inv_v);
       } else {
 #pragma unroll
         for (int e = 0; e < 6; ++e) h[6 * u + e] = 0;
@@ -763,6 +768,31 @@ static int launch_infer(int n, const uint2 *feat4, const int *flags, const float
   return GG_OK;
 }
 
+// A second stream per device (default priority) on which the teacher
+// fallback runs beside the network kernel, with the two events that fork it
+// from and join it back into the caller's stream.  Created on first use,
+// kept for the life of the process.
+struct SideStream {
+  hipStream_t st = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+static int side_stream(SideStream *&out) {
+  static std::mutex mu;
+  static SideStream per_dev[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return GG_ERR_HIP;
+  std::lock_guard<std::mutex> lock(mu);
+  SideStream &s = per_dev[dev];
+  if (!s.st) {
+    if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess) return GG_ERR_HIP;
+    if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess)
+      return GG_ERR_HIP;
+  }
+  out = &s;
+  return GG_OK;
+}
+
 extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
                                const gg_particles *boundary, const int *bcs, gg_grid grid,
                                const gg_upsample_params *u, const gg_mlp_config *c,
@@ -805,6 +835,20 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const uint2 *feat4 = reinterpret_cast<const uint2 *>(feat);
+  // flagged particles: the teacher (the plain upsampler on the template), on
+  // the side stream while the network runs (the two write disjoint outputs)
+  SideStream *side = nullptr;
+  GG_TRY(side_stream(side));
+  if (hipEventRecord(side->fork, stream) != hipSuccess ||
+      hipStreamWaitEvent(side->st, side->fork, 0) != hipSuccess)
+    return GG_ERR_HIP;
+  prof_begin(GG_STAGE_MLP_FALLBACK, side->st);
+  k_mlp_fallback<<<cus * 8, 256, 0, side->st>>>(
+      fb_list, fb_count, pos, vel, cs, reinterpret_cast<const float *>(boundary->pos), bcs, g,
+      up_const(u), c->radius, fine_pos, fine_vel);
+  GG_LAUNCH_CHECK();
+  prof_end(GG_STAGE_MLP_FALLBACK, side->st);
+  if (hipEventRecord(side->join, side->st) != hipSuccess) return GG_ERR_HIP;
   prof_begin(GG_STAGE_MLP, stream);
   int rc;
 #define GG_INFER(W_, T_)                                                                     \
@@ -818,15 +862,12 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   else if (waves == 8 && tiles == 4) GG_INFER(8, 4);
   else GG_INFER(16, 1);
 #undef GG_INFER
-  if (rc != GG_OK) return rc;
+  if (rc != GG_OK) {
+    (void)hipStreamWaitEvent(stream, side->join, 0);
+    return rc;
+  }
   prof_end(GG_STAGE_MLP, stream);
-  // flagged particles: the teacher (the plain upsampler on the template)
-  prof_begin(GG_STAGE_MLP_FALLBACK, stream);
-  k_mlp_fallback<<<cus * 8, 256, 0, stream>>>(
-      fb_list, fb_count, pos, vel, cs, reinterpret_cast<const float *>(boundary->pos), bcs, g,
-      up_const(u), c->radius, fine_pos, fine_vel);
-  GG_LAUNCH_CHECK();
-  prof_end(GG_STAGE_MLP_FALLBACK, stream);
+  if (hipStreamWaitEvent(stream, side->join, 0) != hipSuccess) return GG_ERR_HIP;
   (void)fallback_count;
   return GG_OK;
 }

@@ -44,30 +44,39 @@ __device__ __forceinline__ void upsample_one(
     stencil_rows(bx, by, bz, g, cs, [&](int j0, int j1) {
       int j = j0;
       for (; j + 2 <= j1; j += 2) {
-        const float a0 = cpos[3 * j], a1 = cpos[3 * j + 1], a2 = cpos[3 * j + 2];
-        const float b0 = cpos[3 * j + 3], b1 = cpos[3 * j + 4], b2 = cpos[3 * j + 5];
-        const float v0 = cvel[3 * j], v1 = cvel[3 * j + 1], v2 = cvel[3 * j + 2];
-        const float w0 = cvel[3 * j + 3], w1 = cvel[3 * j + 4], w2 = cvel[3 * j + 5];
-        coarse_term(px, py, pz, a0, a1, a2, v0, v1, v2, u.r9, gw, wx, wy, wz, maxw);
-        coarse_term(px, py, pz, b0, b1, b2, w0, w1, w2, u.r9, gw, wx, wy, wz, maxw);
+        const Packed3 a = ld3(cpos, j), b = ld3(cpos, j + 1);
+        const Packed3 v = ld3(cvel, j), w = ld3(cvel, j + 1);
+        coarse_term(px, py, pz, a.x, a.y, a.z, v.x, v.y, v.z, u.r9, gw, wx, wy, wz, maxw);
+        coarse_term(px, py, pz, b.x, b.y, b.z, w.x, w.y, w.z, u.r9, gw, wx, wy, wz, maxw);
       }
-      for (; j < j1; ++j)
-        coarse_term(px, py, pz, cpos[3 * j], cpos[3 * j + 1], cpos[3 * j + 2], cvel[3 * j],
-                    cvel[3 * j + 1], cvel[3 * j + 2], u.r9, gw, wx, wy, wz, maxw);
+      for (; j < j1; ++j) {
+        const Packed3 a = ld3(cpos, j), v = ld3(cvel, j);
+        coarse_term(px, py, pz, a.x, a.y, a.z, v.x, v.y, v.z, u.r9, gw, wx, wy, wz, maxw);
+      }
     });
-    stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
-      for (int k = k0; k < k1; ++k) {
-        const float tx = px - bpos[3 * k], ty = py - bpos[3 * k + 1], tz = pz - bpos[3 * k + 2];
-        const float dis = sqrtf(tx * tx + ty * ty + tz * tz);
-        if (dis < u.brad) {
-          near = true;
-          const float den = dis + 1e-6f;
-          const float force = (u.rep * (u.brad - dis)) / u.brad;
-          rx = rx + (tx / den) * force;
-          ry = ry + (ty / den) * force;
-          rz = rz + (tz / den) * force;
-        }
+    auto wall = [&](const Packed3 &q) {
+      const float tx = px - q.x, ty = py - q.y, tz = pz - q.z;
+      const float dis = sqrtf(tx * tx + ty * ty + tz * tz);
+      if (dis < u.brad) {
+        near = true;
+        const float den = dis + 1e-6f;
+        const float force = (u.rep * (u.brad - dis)) / u.brad;
+        rx = rx + (tx / den) * force;
+        ry = ry + (ty / den) * force;
+        rz = rz + (tz / den) * force;
       }
+    };
+    stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
+      int k = k0;
+      for (; k + 4 <= k1; k += 4) {
+        const Packed3 q0 = ld3(bpos, k), q1 = ld3(bpos, k + 1), q2 = ld3(bpos, k + 2),
+                      q3 = ld3(bpos, k + 3);
+        wall(q0);
+        wall(q1);
+        wall(q2);
+        wall(q3);
+      }
+      for (; k < k1; ++k) wall(ld3(bpos, k));
     });
   }
   float vx, vy, vz;
