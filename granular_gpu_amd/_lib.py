@@ -136,6 +136,10 @@ _PROTOS = {
     "gg_batch_set_config": (C.c_int, [VP, P(gg_batch_config)]),
     "gg_batch_autotune": (C.c_int, [VP, C.c_int, VP, P(gg_batch_config), P(C.c_float)]),
     "gg_batch_destroy": (None, [VP]),
+    "gg_step_graph_create": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, P(gg_params),
+                                       P(gg_workspace), C.c_int, P(VP)]),
+    "gg_step_graph_launch": (C.c_int, [VP, VP]),
+    "gg_step_graph_destroy": (None, [VP]),
     "gg_mlp_config_default": (None, [P(gg_mlp_config)]),
     "gg_mlp_features": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, gg_grid,
                                   P(gg_mlp_config), VP, VP, VP, VP]),

@@ -306,3 +306,49 @@ extern "C" void gg_batch_destroy(gg_batch *b) {
   release_streams(b);
   delete b;
 }
+
+// ---- captured step -----------------------------------------------------------
+struct gg_step_graph {
+  gg_batch *b;
+};
+
+extern "C" int gg_step_graph_create(const gg_particles *set, int *cell_start,
+                                    const gg_particles *boundary, const int *bcell_start,
+                                    const gg_params *params, const gg_workspace *ws,
+                                    int steps_per_launch, gg_step_graph **out) {
+  if (!out) return GG_ERR_INVALID;
+  *out = nullptr;
+  if (!set || !cell_start || !boundary || !bcell_start || !params || !ws) return GG_ERR_INVALID;
+  gg_batch_scene sc;
+  sc.set = *set;
+  sc.cell_start = cell_start;
+  sc.boundary = *boundary;
+  sc.bcell_start = bcell_start;
+  sc.params = *params;
+  sc.ws = *ws;
+  gg_batch_config c;
+  c.branches = 1;
+  c.sweep_block = 0;
+  c.nlist_block = 0;
+  c.steps_per_launch = steps_per_launch;
+  gg_batch *b = nullptr;
+  GG_TRY(gg_batch_create(&sc, 1, &c, &b));
+  gg_step_graph *g = new (std::nothrow) gg_step_graph{b};
+  if (!g) {
+    gg_batch_destroy(b);
+    return GG_ERR_HIP;
+  }
+  *out = g;
+  return GG_OK;
+}
+
+extern "C" int gg_step_graph_launch(gg_step_graph *g, hipStream_t stream) {
+  if (!g) return GG_ERR_INVALID;
+  return gg_batch_step(g->b, stream);
+}
+
+extern "C" void gg_step_graph_destroy(gg_step_graph *g) {
+  if (!g) return;
+  gg_batch_destroy(g->b);
+  delete g;
+}

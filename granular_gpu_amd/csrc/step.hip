@@ -609,7 +609,7 @@ static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, cons
   prof_end(GG_STAGE_PREDICT, st);
   if (lists_on(p, ws) && n > 0) {
     prof_begin(GG_STAGE_NLIST, st);
-    GG_TRY(launch_nlist(block_of(p->nlist_block), n, at<float4>(ws, L.x0), at<float4>(ws, L.off1),
+    GG_TRY(launch_nlist(p->nlist_block == 0 ? 64 : p->nlist_block, n, at<float4>(ws, L.x0), at<float4>(ws, L.off1),
                         at<float4>(ws, L.bnd4), cs, bcs, dev_grid(ws->grid), sweep_const(p, ws),
                         p->skin, at<int4>(ws, L.meta), at<int>(ws, L.glist),
                         at<int>(ws, L.blist), (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1, st));

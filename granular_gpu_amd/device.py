@@ -102,6 +102,10 @@ class PlainSim:
                                               C.byref(self.bset), _ptr(self.bcs),
                                               C.byref(self.params), C.byref(self.ws), _stream()))
 
+    def graph(self, steps_per_launch=1) -> "StepGraph":
+        """gg_step of this scene captured into a hipGraph (gg_step_graph)."""
+        return StepGraph(self, steps_per_launch)
+
     # stage-level entry points ------------------------------------------------
     def grid_build(self, perm_out: torch.Tensor | None = None):
         check("gg_grid_build", self.lib.gg_grid_build(C.byref(self.set), None, _ptr(self.cs),
@@ -249,6 +253,34 @@ class FineSet:
     def host(self):
         torch.cuda.synchronize()
         return self.pos.cpu().numpy().copy(), self.vel.cpu().numpy().copy()
+
+
+class StepGraph:
+    """A captured fixed-N step of one PlainSim (gg_step_graph_*): each launch
+    advances the scene by steps_per_launch steps."""
+
+    def __init__(self, sim: PlainSim, steps_per_launch=1):
+        self.lib, self.sim = sim.lib, sim
+        h = C.c_void_p()
+        check("gg_step_graph_create", self.lib.gg_step_graph_create(
+            C.byref(sim.set), _ptr(sim.cs), C.byref(sim.bset), _ptr(sim.bcs),
+            C.byref(sim.params), C.byref(sim.ws), steps_per_launch, C.byref(h)))
+        self.h = h
+
+    def launch(self, n=1):
+        for _ in range(n):
+            check("gg_step_graph_launch", self.lib.gg_step_graph_launch(self.h, _stream()))
+
+    def close(self):
+        if self.h:
+            self.lib.gg_step_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BatchSim:

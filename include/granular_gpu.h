@@ -74,7 +74,7 @@ typedef struct gg_params {
   int sweep_block;          /* threads per block of the contact sweeps:
                                64, 128, 256 or 512; 0 = 64 (autotuned by
                                gg_batch_autotune)                           */
-  int nlist_block;          /* same for the neighbour-list build; 0 = 256   */
+  int nlist_block;          /* same for the neighbour-list build; 0 = 64    */
 } gg_params;
 
 #define GG_FLAG_NO_NLIST 1     /* always scan the 27-cell stencil */
@@ -418,6 +418,19 @@ int gg_batch_set_config(gg_batch *b, const gg_batch_config *cfg);
 int gg_batch_autotune(gg_batch *b, int trials, hipStream_t stream, gg_batch_config *best,
                       float *best_ms);
 void gg_batch_destroy(gg_batch *b);
+
+/* ---- captured step -------------------------------------------------------- */
+/* gg_step of one fixed-N plain scene (no merge/split) captured into a
+ * hipGraph: steps_per_launch (1..1000) steps per launch, replayed without
+ * per-kernel launch overhead.  The set's arrays and n, the cell tables, the
+ * params and the workspace are captured at creation; after changing any of
+ * them, destroy and create again.  A one-scene gg_batch underneath. */
+typedef struct gg_step_graph gg_step_graph;
+int gg_step_graph_create(const gg_particles *set, int *cell_start, const gg_particles *boundary,
+                         const int *bcell_start, const gg_params *params,
+                         const gg_workspace *ws, int steps_per_launch, gg_step_graph **out);
+int gg_step_graph_launch(gg_step_graph *g, hipStream_t stream);
+void gg_step_graph_destroy(gg_step_graph *g);
 
 /* ---- device-array primitives (host DArray class) --------------------------- */
 /* Stable compaction (DArray<T>::compact, DArray.hpp:62-122): copies the

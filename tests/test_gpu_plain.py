@@ -174,6 +174,28 @@ def test_fallback_paths_agree(gpu):
     assert np.array_equal(runs["wave"][0][1], ov)
 
 
+def test_step_graph_replays_equal_eager_steps(gpu):
+    # every replay of the captured step (eager work and a synchronisation
+    # between replays) must equal the same steps launched kernel by kernel
+    import torch
+    _log("captured step (gg_step_graph)")
+    s = scenes.plain_box(32, 1.0)
+    a = _sim(s, stab=3, solver=10, mu_s=0.5, mu_k=0.4)
+    b = _sim(s, stab=3, solver=10, mu_s=0.5, mu_k=0.4)
+    g = b.graph(steps_per_launch=5)
+    for r in range(4):
+        g.launch()
+        scratch = torch.ones(1 << 20, device="cuda").cumsum(0)  # eager work in between
+        torch.cuda.synchronize()
+        a.step(5)
+        assert float(scratch[-1].item()) == float(1 << 20)
+        for x, y in zip(a.host(), b.host()):
+            assert np.array_equal(x, y), r
+    g.close()
+    op, ov, _, _, _, _ = _oracle_steps(s, 20, 3, 10, 0.5, 0.4)
+    assert np.array_equal(b.host()[0], op) and np.array_equal(b.host()[1], ov)
+
+
 def test_run_twice_bitwise_identical(gpu):
     _log("determinism")
     s = scenes.plain_box(32, 1.0)
