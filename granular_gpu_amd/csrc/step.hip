@@ -157,25 +157,34 @@ __global__ void __launch_bounds__(BS)
         }
       }
     });
-    auto test = [&](int j, const float4 &sj) {
+    // 1 if j (at x* sj) is a candidate of i (self excluded below)
+    auto near = [&](const float4 &sj) -> unsigned int {
       const float px = si.x - sj.x, py = si.y - sj.y, pz = si.z - sj.z;
       const float d2 = px * px + py * py + pz * pz;
       const float lim = (si.w + sj.w) + skin;
-      if (d2 < lim * lim && j != i) {
-        if (gc < sc.gmax) glist[(size_t)gc * sc.cap + i] = j;
-        ++gc;
-      }
+      return d2 < lim * lim ? 1u : 0u;
     };
+    // Rows in chunks of 32 candidates: the tests only set bits of a mask;
+    // the (few) candidates found are appended afterwards in index order.
     stencil_rows(bx, by, bz, g, cs, [&](int j0, int j1) {
-      int j = j0;
-      for (; j + 4 <= j1; j += 4) {
-        const float4 s0 = xs[j], s1 = xs[j + 1], s2 = xs[j + 2], s3 = xs[j + 3];
-        test(j, s0);
-        test(j + 1, s1);
-        test(j + 2, s2);
-        test(j + 3, s3);
+      for (int c0 = j0; c0 < j1; c0 += 32) {
+        const int cn = j1 - c0 < 32 ? j1 - c0 : 32;
+        unsigned int mask = 0u;
+        int t = 0;
+        for (; t + 4 <= cn; t += 4) {
+          const float4 s0 = xs[c0 + t], s1 = xs[c0 + t + 1], s2 = xs[c0 + t + 2],
+                       s3 = xs[c0 + t + 3];
+          mask |= (near(s0) | (near(s1) << 1) | (near(s2) << 2) | (near(s3) << 3)) << t;
+        }
+        for (; t < cn; ++t) mask |= near(xs[c0 + t]) << t;
+        if (i >= c0 && i < c0 + cn) mask &= ~(1u << (i - c0));
+        while (mask) {
+          const int b = __ffs((int)mask) - 1;
+          mask &= mask - 1u;
+          if (gc < sc.gmax) glist[(size_t)gc * sc.cap + i] = c0 + b;
+          ++gc;
+        }
       }
-      for (; j < j1; ++j) test(j, xs[j]);
     });
   }
   const float4 bi = x[i];
