@@ -13,7 +13,7 @@ namespace gg {
 constexpr int kBlock = 256;        // threads per block for particle kernels
 constexpr int kScanTile = 2048;    // elements per block in the cell-table scan
 constexpr int kScanMaxParts = 8192;
-constexpr int kSmallCell = 32;     // cells up to this size are sorted per thread
+constexpr int kSmallCell = 32;     // cells up to this size are ranked one thread per entry
 constexpr int kBigLds = 4096;      // cells up to this size are sorted in LDS
 constexpr double kPiD = 3.14159265358979323846;
 constexpr float kPiF = 3.14159265358979323846f;
@@ -33,7 +33,7 @@ struct Layout {
   size_t key;      // int[cmax]      cell id per particle (input order)
   size_t slot;     // int[cmax]      arrival slot inside the cell
   size_t perm;     // int[cmax]      sorted slot -> old index
-  size_t itmp;     // int[cmax]      big-cell sort scratch
+  size_t itmp;     // int[cmax]      cell order as scattered (before the stable ranking)
   size_t part;     // int[parts+1]   scan partials
   size_t big;      // int[C+1]       list of cells needing the big sort
   size_t ctr;      // int[64]        device counters ([0] = big-cell count)

@@ -165,30 +165,30 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const int *__restrict__ key,
   if (d >= 0 && d < n) perm[d] = i;
 }
 
-// One thread per cell: insertion sort of small cells by old index; larger
-// cells are queued for k_sort_big.
-__global__ void __launch_bounds__(kBlock) k_sort_small(const int *__restrict__ cell_start, int m,
+// One thread per sorted slot d: raw[d] (the scatter's arrival order) goes
+// to cell_start[c] + (number of entries of its cell c with a smaller old
+// index), i.e. ascending old index within the cell.  Cells longer than
+// kSmallCell are copied as they are and queued (by their first slot) for
+// k_sort_big.
+__global__ void __launch_bounds__(kBlock) k_rank_small(const int *__restrict__ raw, int n,
+                                                       const int *__restrict__ key,
+                                                       const int *__restrict__ cell_start,
                                                        int *__restrict__ perm,
                                                        int *__restrict__ big,
                                                        int *__restrict__ ctr) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= m) return;
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  const int v = raw[d];
+  const int c = key[v];
   const int s = cell_start[c], e = cell_start[c + 1];
-  const int len = e - s;
-  if (len <= 1) return;
-  if (len > kSmallCell) {
-    big[atomicAdd(&ctr[0], 1)] = c;
+  if (e - s > kSmallCell) {
+    perm[d] = v;
+    if (d == s) big[atomicAdd(&ctr[0], 1)] = c;
     return;
   }
-  for (int a = s + 1; a < e; ++a) {
-    const int v = perm[a];
-    int b = a - 1;
-    while (b >= s && perm[b] > v) {
-      perm[b + 1] = perm[b];
-      --b;
-    }
-    perm[b + 1] = v;
-  }
+  int rank = 0;
+  for (int k = s; k < e; ++k) rank += raw[k] < v ? 1 : 0;
+  perm[s + rank] = v;
 }
 
 // One block per queued cell (grid-stride over the queue).  Cells up to
@@ -325,11 +325,12 @@ int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *w
   k_scan_apply<<<np, kBlock, 0, st>>>(count, m, part, cell_start);
   GG_LAUNCH_CHECK();
   if (n > 0) {
-    k_scatter<<<blocks_for(n), kBlock, 0, st>>>(key, slot, n, cell_start, perm);
+    int *raw = at<int>(ws, L.itmp);
+    k_scatter<<<blocks_for(n), kBlock, 0, st>>>(key, slot, n, cell_start, raw);
+    GG_LAUNCH_CHECK();
+    k_rank_small<<<blocks_for(n), kBlock, 0, st>>>(raw, n, key, cell_start, perm, big, ctr);
     GG_LAUNCH_CHECK();
   }
-  k_sort_small<<<blocks_for(m), kBlock, 0, st>>>(cell_start, m, perm, big, ctr);
-  GG_LAUNCH_CHECK();
   k_sort_big<<<256, kBlock, 0, st>>>(cell_start, perm, big, ctr, key, n);
   GG_LAUNCH_CHECK();
   return GG_OK;
