@@ -105,6 +105,9 @@ struct Packed3 {
 __device__ __forceinline__ Packed3 ld3(const float *__restrict__ p, long long i) {
   return reinterpret_cast<const Packed3 *>(p)[i];
 }
+__device__ __forceinline__ void st3(float *__restrict__ p, long long i, float x, float y, float z) {
+  reinterpret_cast<Packed3 *>(p)[i] = Packed3{x, y, z};
+}
 
 // exp / cbrt evaluated in double and rounded once to float.
 __device__ __forceinline__ float exp_f(float x) { return (float)exp((double)x); }

@@ -27,8 +27,8 @@ __global__ void __launch_bounds__(kBlock) k_hash_count(const float *__restrict__
   // All lanes take part in the wave operations; lanes past n carry key -1.
   int c = -1;
   if (i < n) {
-    const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
-    c = hash_cell(px, py, pz, g);
+    const Packed3 p3 = ld3(pos, i);
+    c = hash_cell(p3.x, p3.y, p3.z, g);
     key[i] = c;
   }
   // Wave-aggregated slot reservation: each run of equal keys in adjacent

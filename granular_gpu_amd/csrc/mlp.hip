@@ -58,6 +58,20 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+
+// v_mfma_f32_16x16x32_bf16: lane l = c + 16 q holds 8 elements of row c of
+// A and of column c of B at k-slots owned by q, and D[4q..4q+3][c].
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8v, a),
+                                                 __builtin_bit_cast(bf16x8v, b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 cat8(bf16x4 lo, bf16x4 hi) {
+  return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
 __device__ __forceinline__ float templ(int k, int axis) {
   return ((k >> (2 - axis)) & 1) ? 0.5f : -0.5f;
 }
@@ -150,28 +164,30 @@ __global__ void __launch_bounds__(256)
   // two neighbour slots = 12 features = 3 groups of 4
 #pragma unroll
   for (int s = 0; s < kKnn; s += 2) {
-    uint16_t h[12];
+    float f[12];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int j = kj[s + u];
       if (knn) knn[(size_t)i * kKnn + s + u] = j;
       if (j >= 0) {
         const Packed3 pj = ld3(pos, j), vj = ld3(vel, j);
-        h[6 * u + 0] = to_bf16((pj.x - xx) * inv_r);
-        h[6 * u + 1] = to_bf16((pj.y - xy) * inv_r);
-        h[6 * u + 2] = to_bf16((pj.z - xz) * inv_r);
-        h[6 * u + 3] = to_bf16((vj.x - vx) * inv_v);
-        h[6 * u + 4] = to_bf16((vj.y - vy) * inv_v);
-        h[6 * u + 5] = to_bf16((vj.z - vz) * inv_v);
+        f[6 * u + 0] = (pj.x - xx) * inv_r;
+        f[6 * u + 1] = (pj.y - xy) * inv_r;
+        f[6 * u + 2] = (pj.z - xz) * inv_r;
+        f[6 * u + 3] = (vj.x - vx) * inv_v;
+        f[6 * u + 4] = (vj.y - vy) * inv_v;
+        f[6 * u + 5] = (vj.z - vz) * inv_v;
       } else {
 #pragma unroll
-        for (int e = 0; e < 6; ++e) h[6 * u + e] = 0;
+        for (int e = 0; e < 6; ++e) f[6 * u + e] = 0.0f;
       }
     }
+    // hardware round-to-nearest-even (the defined rounding on every non-NaN
+    // value; a NaN stays a NaN)
 #pragma unroll
     for (int gq = 0; gq < 3; ++gq) {
-      const uint2 w = make_uint2((uint32_t)h[4 * gq] | ((uint32_t)h[4 * gq + 1] << 16),
-                                 (uint32_t)h[4 * gq + 2] | ((uint32_t)h[4 * gq + 3] << 16));
+      const uint2 w = make_uint2(pack_bf16x2(f[4 * gq], f[4 * gq + 1]),
+                                 pack_bf16x2(f[4 * gq + 2], f[4 * gq + 3]));
       const int grp = 3 * (s / 2) + gq;  // features 4 grp .. 4 grp + 3
       if (GROUPED) {
         reinterpret_cast<uint2 *>(feat)[(size_t)grp * n + i] = w;
@@ -377,58 +393,67 @@ __global__ void k_to_bf16(long long count, const float *__restrict__ x, uint16_t
 // live in LDS, shared by the waves.  A wave takes 16 TILES samples per
 // iteration: TILES MFMA column tiles that share every weight fragment.
 
+// The inference kernel uses v_mfma_f32_16x16x32_bf16.  A k-step covers 32
+// features, and element e of lane (c, q) stands for feature
+//   f(ks, q, e) = 32 ks + 16 (e / 4) + 4 q + e % 4,
+// which is exactly where the previous layer's output tiles 2ks (e < 4) and
+// 2ks + 1 (e >= 4) leave that feature in lane (c, q): activations go from
+// one layer to the next without any cross-lane move.  The weights use the
+// same k order (a sum over all k does not depend on it).
+//
 // Weights in LDS are stored in fragment order: the A fragment of output
-// tile t and k-step ks for lane l is the 4 bfloat16 at
-// ((t * KSTEPS + ks) * 64 + l) * 4, so a wave's fragment read is one
-// contiguous 512-byte block (no bank conflicts).  Row-major W[o][k] maps to
-// t = o / 16, ks = k / 16, l = o % 16 + 16 ((k % 16) / 4), e = k % 4.
-__device__ __forceinline__ const bf16x4 &frag(const uint16_t *__restrict__ W, int ksteps, int t,
+// tile t and k-step ks for lane l is the 8 bfloat16 at
+// ((t * KSTEPS + ks) * 64 + l) * 8, so a wave's fragment read is one
+// contiguous 1 KB block (no bank conflicts).
+__device__ __forceinline__ const bf16x8 &frag(const uint16_t *__restrict__ W, int ksteps, int t,
                                               int ks, int lane) {
-  return *reinterpret_cast<const bf16x4 *>(W + ((t * ksteps + ks) * 64 + lane) * 4);
+  return *reinterpret_cast<const bf16x8 *>(W + ((t * ksteps + ks) * 64 + lane) * 8);
 }
 
-// Copies one layer's row-major weights (rows x 16*ksteps) into fragment order.
+// Copies one layer's row-major weights (rows x 32*ksteps) into fragment order.
 __device__ __forceinline__ void stage_layer(uint16_t *__restrict__ dst,
                                             const uint16_t *__restrict__ src, int rows,
                                             int ksteps) {
-  const int frags = (rows / 16) * ksteps * 64;  // 4-element fragments
+  const int frags = (rows / 16) * ksteps * 64;  // 8-element fragments
   for (int f = threadIdx.x; f < frags; f += blockDim.x) {
     const int lane = f & 63, fk = f >> 6;
     const int t = fk / ksteps, ks = fk % ksteps;
-    const int o = 16 * t + (lane & 15), k = 16 * ks + 4 * (lane >> 4);
-    reinterpret_cast<uint2 *>(dst)[f] =
-        *reinterpret_cast<const uint2 *>(src + (size_t)o * (16 * ksteps) + k);
+    const int o = 16 * t + (lane & 15), k = 32 * ks + 4 * (lane >> 4);
+    const uint16_t *row = src + (size_t)o * (32 * ksteps);
+    const uint2 lo = *reinterpret_cast<const uint2 *>(row + k);
+    const uint2 hi = *reinterpret_cast<const uint2 *>(row + k + 16);
+    reinterpret_cast<uint4 *>(dst)[f] = make_uint4(lo.x, lo.y, hi.x, hi.y);
   }
 }
 
 // One layer, transposed: output tile t (features 16t..16t+15) of every
-// sample tile from KSTEPS input k-steps; bias, ReLU and bfloat16 packing
-// give the next layer's B operand directly.
+// sample tile from KSTEPS 32-wide k-steps, accumulated onto the bias; ReLU
+// and hardware bfloat16 rounding give the next layer's operand halves
+// (out[s][t] = features 16t + 4q .. + 3 of lane (c, q)'s sample).
 template <int TILES, int KSTEPS>
 __device__ __forceinline__ void hidden_layer(const uint16_t *__restrict__ W,
                                              const float *__restrict__ bias,
-                                             const bf16x4 (&in)[TILES][8],
+                                             const bf16x8 (&in)[TILES][4],
                                              bf16x4 (&out)[TILES][8], int lane) {
   const int q = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
+    const f32x4 b = *reinterpret_cast<const f32x4 *>(bias + 16 * t + 4 * q);
     f32x4 a[TILES];
 #pragma unroll
-    for (int s = 0; s < TILES; ++s) a[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int s = 0; s < TILES; ++s) a[s] = b;
 #pragma unroll
     for (int ks = 0; ks < KSTEPS; ++ks) {
-      const bf16x4 w = frag(W, KSTEPS, t, ks, lane);
+      const bf16x8 w = frag(W, KSTEPS, t, ks, lane);
 #pragma unroll
-      for (int s = 0; s < TILES; ++s) a[s] = mfma16(w, in[s][ks], a[s]);
+      for (int s = 0; s < TILES; ++s) a[s] = mfma32(w, in[s][ks], a[s]);
     }
-    // bias + ReLU (no NaN survives it), then hardware round-to-nearest-even
-    const float b0 = bias[16 * t + 4 * q], b1 = bias[16 * t + 4 * q + 1];
-    const float b2 = bias[16 * t + 4 * q + 2], b3 = bias[16 * t + 4 * q + 3];
+    // ReLU (no NaN survives it), then hardware round-to-nearest-even
 #pragma unroll
     for (int s = 0; s < TILES; ++s) {
-      const float z0 = a[s][0] + b0, z1 = a[s][1] + b1, z2 = a[s][2] + b2, z3 = a[s][3] + b3;
-      const u32x2v h = {pack_bf16x2(z0 > 0.0f ? z0 : 0.0f, z1 > 0.0f ? z1 : 0.0f),
-                        pack_bf16x2(z2 > 0.0f ? z2 : 0.0f, z3 > 0.0f ? z3 : 0.0f)};
+      const f32x4 z = a[s];
+      const u32x2v h = {pack_bf16x2(z[0] > 0.0f ? z[0] : 0.0f, z[1] > 0.0f ? z[1] : 0.0f),
+                        pack_bf16x2(z[2] > 0.0f ? z[2] : 0.0f, z[3] > 0.0f ? z[3] : 0.0f)};
       out[s][t] = __builtin_bit_cast(bf16x4, h);
     }
     // keep each tile's weight loads next to its MFMAs (bounds live registers)
@@ -436,29 +461,32 @@ __device__ __forceinline__ void hidden_layer(const uint16_t *__restrict__ W,
   }
 }
 
-// Output layer: 2 tiles (24 outputs + 8 zero rows), no activation.
+// The next layer's k-step operands from a hidden layer's output tiles.
+template <int TILES>
+__device__ __forceinline__ void operands(const bf16x4 (&h)[TILES][8], bf16x8 (&in)[TILES][4]) {
+#pragma unroll
+  for (int s = 0; s < TILES; ++s)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) in[s][ks] = cat8(h[s][2 * ks], h[s][2 * ks + 1]);
+}
+
+// Output layer: 2 tiles (24 outputs + 8 zero rows) onto the bias, no activation.
 template <int TILES>
 __device__ __forceinline__ void output_layer(const uint16_t *__restrict__ W,
                                              const float *__restrict__ bias,
-                                             const bf16x4 (&in)[TILES][8],
+                                             const bf16x8 (&in)[TILES][4],
                                              f32x4 (&y)[TILES][2], int lane) {
   const int q = lane >> 4;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
-    f32x4 a[TILES];
+    const f32x4 b = *reinterpret_cast<const f32x4 *>(bias + 16 * t + 4 * q);
 #pragma unroll
-    for (int s = 0; s < TILES; ++s) a[s] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int s = 0; s < TILES; ++s) y[s][t] = b;
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      const bf16x4 w = frag(W, 8, t, ks, lane);
+    for (int ks = 0; ks < 4; ++ks) {
+      const bf16x8 w = frag(W, 4, t, ks, lane);
 #pragma unroll
-      for (int s = 0; s < TILES; ++s) a[s] = mfma16(w, in[s][ks], a[s]);
-    }
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float b = bias[16 * t + 4 * q + e];
-#pragma unroll
-      for (int s = 0; s < TILES; ++s) y[s][t][e] = a[s][e] + b;
+      for (int s = 0; s < TILES; ++s) y[s][t] = mfma32(w, in[s][ks], y[s][t]);
     }
   }
 }
@@ -471,11 +499,11 @@ __global__ void __launch_bounds__(64 * WAVES)
                 const uint16_t *__restrict__ wts, const float *__restrict__ bias_g, float radius,
                 float dt, float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
   __shared__ __attribute__((aligned(16))) uint16_t W[49152];
-  __shared__ float bias[416];
-  stage_layer(W + kWOff1, wts + kWOff1, kHid, kIn / 16);
-  stage_layer(W + kWOff2, wts + kWOff2, kHid, kHid / 16);
-  stage_layer(W + kWOff3, wts + kWOff3, kHid, kHid / 16);
-  stage_layer(W + kWOff4, wts + kWOff4, kOutPad, kHid / 16);
+  __shared__ __attribute__((aligned(16))) float bias[416];
+  stage_layer(W + kWOff1, wts + kWOff1, kHid, kIn / 32);
+  stage_layer(W + kWOff2, wts + kWOff2, kHid, kHid / 32);
+  stage_layer(W + kWOff3, wts + kWOff3, kHid, kHid / 32);
+  stage_layer(W + kWOff4, wts + kWOff4, kOutPad, kHid / 32);
   for (int i = threadIdx.x; i < 416; i += blockDim.x) bias[i] = bias_g[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -484,26 +512,32 @@ __global__ void __launch_bounds__(64 * WAVES)
   const int groups = (n + per_iter - 1) / per_iter;
   for (int grp = blockIdx.x * WAVES + wave; grp < groups; grp += gridDim.x * WAVES) {
     const int base = grp * per_iter;
-    bf16x4 h[TILES][8], h2[TILES][8];
+    bf16x8 in[TILES][4];
+    bf16x4 h[TILES][8];
     f32x4 y[TILES][2];
-    // layer 1 input: features 16 ks + 4 q + e of sample base + 16 s + c
+    // layer 1 operands: k-step ks of lane (c, q) = feature groups 8 ks + q
+    // and 8 ks + 4 + q of sample base + 16 s + c
 #pragma unroll
     for (int s = 0; s < TILES; ++s) {
       const int smp = base + 16 * s + c;
 #pragma unroll
-      for (int ks = 0; ks < 6; ++ks) {
-        uint2 w = make_uint2(0u, 0u);
-        if (smp < n) w = feat4[(size_t)(4 * ks + q) * n + smp];
-        h[s][ks] = bf16x4{(short)(w.x & 0xffffu), (short)(w.x >> 16), (short)(w.y & 0xffffu),
-                          (short)(w.y >> 16)};
+      for (int ks = 0; ks < 3; ++ks) {
+        uint2 lo = make_uint2(0u, 0u), hi = make_uint2(0u, 0u);
+        if (smp < n) {
+          lo = feat4[(size_t)(8 * ks + q) * n + smp];
+          hi = feat4[(size_t)(8 * ks + 4 + q) * n + smp];
+        }
+        in[s][ks] = __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
       }
-      h[s][6] = bf16x4{0, 0, 0, 0};
-      h[s][7] = bf16x4{0, 0, 0, 0};
+      in[s][3] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
     }
-    hidden_layer<TILES, 6>(W + kWOff1, bias + kBOff1, h, h2, lane);
-    hidden_layer<TILES, 8>(W + kWOff2, bias + kBOff2, h2, h, lane);
-    hidden_layer<TILES, 8>(W + kWOff3, bias + kBOff3, h, h2, lane);
-    output_layer<TILES>(W + kWOff4, bias + kBOff4, h2, y, lane);
+    hidden_layer<TILES, 3>(W + kWOff1, bias + kBOff1, in, h, lane);
+    operands<TILES>(h, in);
+    hidden_layer<TILES, 4>(W + kWOff2, bias + kBOff2, in, h, lane);
+    operands<TILES>(h, in);
+    hidden_layer<TILES, 4>(W + kWOff3, bias + kBOff3, in, h, lane);
+    operands<TILES>(h, in);
+    output_layer<TILES>(W + kWOff4, bias + kBOff4, in, y, lane);
     // outputs f = 16 t + 4 q + e of sample base + 16 s + c
 #pragma unroll
     for (int s = 0; s < TILES; ++s) {

@@ -53,8 +53,9 @@ __global__ void __launch_bounds__(kBlock)
   float dmax = 0.0f, rmax = 0.0f;
   if (i < n) {
     const int o = perm ? perm[i] : i;
-    const float px = pos[3 * o], py = pos[3 * o + 1], pz = pos[3 * o + 2];
-    float vx = vel[3 * o], vy = vel[3 * o + 1], vz = vel[3 * o + 2];
+    const Packed3 p3 = ld3(pos, o), v3 = ld3(vel, o);
+    const float px = p3.x, py = p3.y, pz = p3.z;
+    float vx = v3.x, vy = v3.y, vz = v3.z;
     vx = vx + dvx;
     vy = vy + dvy;
     vz = vz + dvz;
@@ -524,12 +525,8 @@ __global__ void __launch_bounds__(kBlock)
       vz = vz * fc.adaptive_damping;
     }
   }
-  vel[3 * i] = vx;
-  vel[3 * i + 1] = vy;
-  vel[3 * i + 2] = vz;
-  pos[3 * i] = b.x + o.x;
-  pos[3 * i + 1] = b.y + o.y;
-  pos[3 * i + 2] = b.z + o.z;
+  st3(vel, i, vx, vy, vz);
+  st3(pos, i, b.x + o.x, b.y + o.y, b.z + o.z);
   if (adl) adl[i] = 0;
   if (mass) mass[i] = mass_tmp[i];
   if (rem) rem[i] = rem_tmp[i];
@@ -539,8 +536,8 @@ __global__ void __launch_bounds__(kBlock)
 
 static inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-// Threads per block of the list build and the sweeps (0 = 256).
-static inline int block_of(int b) { return b == 0 ? 256 : b; }
+// Threads per block of the list build and the sweeps: 0 (the default, 64) or
+// one of these.
 static inline bool block_ok(int b) { return b == 0 || b == 64 || b == 128 || b == 256 || b == 512; }
 
 static int check_params(const gg_params *p, const gg_workspace *ws) {
