@@ -80,6 +80,56 @@ struct MlpConst {
   float radius, vscale, guard_boundary, guard_floor, guard_side;
 };
 
+// ---- fallback flags -----------------------------------------------------------
+// flag bit 1: a boundary particle of the stencil within guard_boundary;
+// bit 2: below guard_floor or near the x = 0 / z = 0 planes.  Flagged
+// particles are appended to a compacted list (order is irrelevant: each one
+// writes only its own outputs).
+__global__ void __launch_bounds__(256)
+    k_mlp_flags(int n, const float *__restrict__ pos, const float *__restrict__ bpos,
+                const int *__restrict__ bcs, DevGrid g, MlpConst c, int *__restrict__ flags,
+                int *__restrict__ fb_list, int *__restrict__ fb_count) {
+  const int i = xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x;
+  int flag = 0;
+  if (i < n) {
+    const Packed3 xi = ld3(pos, i);
+    const float gb2 = c.guard_boundary * c.guard_boundary;
+    int bx, by, bz;
+    if (floor_cell(xi.x, xi.y, xi.z, g, bx, by, bz)) {
+      stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
+        int k = k0;
+        for (; k + 4 <= k1; k += 4) {
+          const Packed3 q0 = ld3(bpos, k), q1 = ld3(bpos, k + 1), q2 = ld3(bpos, k + 2),
+                        q3 = ld3(bpos, k + 3);
+          auto near = [&](const Packed3 &q) {
+            const float dx = q.x - xi.x, dy = q.y - xi.y, dz = q.z - xi.z;
+            return dx * dx + dy * dy + dz * dz < gb2;
+          };
+          if (near(q0) || near(q1) || near(q2) || near(q3)) flag |= 1;
+        }
+        for (; k < k1; ++k) {
+          const Packed3 q = ld3(bpos, k);
+          const float dx = q.x - xi.x, dy = q.y - xi.y, dz = q.z - xi.z;
+          if (dx * dx + dy * dy + dz * dz < gb2) flag |= 1;
+        }
+      });
+    }
+    if (xi.y < c.guard_floor || xi.x < c.guard_side || xi.z < c.guard_side) flag |= 2;
+    flags[i] = flag;
+  }
+  if (fb_list) {
+    const unsigned long long m = __ballot(flag != 0);
+    if (flag) {
+      const int lane = threadIdx.x & 63;
+      const int leader = __ffsll((long long)m) - 1;
+      int slot = 0;
+      if (lane == leader) slot = atomicAdd(fb_count, __popcll(m));
+      slot = __shfl(slot, leader, 64) + __popcll(m & ((1ull << lane) - 1ull));
+      fb_list[slot] = i;
+    }
+  }
+}
+
 // ---- features ---------------------------------------------------------------
 // Stencil cells nearest-first (centre, 6 faces, 12 edges, 8 corners) as
 // stencil_cell indices m = 9 (dx+1) + 3 (dy+1) + (dz+1).  The 16 smallest
@@ -98,10 +148,8 @@ __constant__ int kNearFirst[27] = {13, 4,  10, 12, 14, 16, 22, 1,  3,  5,  7,  9
 template <int GROUPED>
 __global__ void __launch_bounds__(256)
     k_mlp_features(int n, const float *__restrict__ pos, const float *__restrict__ vel,
-                   const int *__restrict__ cs, const float *__restrict__ bpos,
-                   const int *__restrict__ bcs, DevGrid g, MlpConst c,
-                   uint16_t *__restrict__ feat, int *__restrict__ flags, int *__restrict__ knn,
-                   int *__restrict__ fb_list, int *__restrict__ fb_count) {
+                   const int *__restrict__ cs, DevGrid g, MlpConst c,
+                   uint16_t *__restrict__ feat, int *__restrict__ knn) {
   // consecutive blocks stay on one XCD (spatial neighbours share its L2)
   const int i = xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -111,8 +159,6 @@ __global__ void __launch_bounds__(256)
   unsigned long long key[kKnn];
 #pragma unroll
   for (int s = 0; s < kKnn; ++s) key[s] = kNone;
-  int flag = 0;
-  const float gb2 = c.guard_boundary * c.guard_boundary;
   int bx, by, bz;
   if (floor_cell(xx, xy, xz, g, bx, by, bz)) {
     for (int m = 0; m < 27; ++m) {
@@ -134,30 +180,10 @@ __global__ void __launch_bounds__(256)
         key[0] = lt[0] ? k : key[0];
       }
     }
-    stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
-      for (int k = k0; k < k1; ++k) {
-        const Packed3 q = ld3(bpos, k);
-        const float dx = q.x - xx, dy = q.y - xy, dz = q.z - xz;
-        if (dx * dx + dy * dy + dz * dz < gb2) flag |= 1;
-      }
-    });
   }
   int kj[kKnn];
 #pragma unroll
   for (int s = 0; s < kKnn; ++s) kj[s] = key[s] == kNone ? -1 : (int)(unsigned)(key[s] & 0xffffffffu);
-  if (xy < c.guard_floor || xx < c.guard_side || xz < c.guard_side) flag |= 2;
-  flags[i] = flag;
-  if (flag && fb_list) {
-    // compacted list of flagged particles (order is irrelevant: each one
-    // writes only its own outputs)
-    const unsigned long long m = __ballot(1);
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)m) - 1;
-    int slot = 0;
-    if (lane == leader) slot = atomicAdd(fb_count, __popcll(m));
-    slot = __shfl(slot, leader, 64) + __popcll(m & ((1ull << lane) - 1ull));
-    fb_list[slot] = i;
-  }
   const Packed3 vi = ld3(vel, i);
   const float vx = vi.x, vy = vi.y, vz = vi.z;
   const float inv_r = 1.0f / c.radius, inv_v = 1.0f / c.vscale;
@@ -678,11 +704,14 @@ extern "C" int gg_mlp_features(const gg_particles *coarse, const int *cs,
   if (!(c->radius > 0.0f) || !(c->vscale > 0.0f)) return GG_ERR_INVALID;
   if (coarse->n == 0) return GG_OK;
   if (!coarse->pos || !coarse->vel || (boundary->n > 0 && !boundary->pos)) return GG_ERR_INVALID;
+  k_mlp_flags<<<nblocks(coarse->n, 256), 256, 0, stream>>>(
+      coarse->n, reinterpret_cast<const float *>(coarse->pos),
+      reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), mlp_const(c), flags,
+      nullptr, nullptr);
+  GG_LAUNCH_CHECK();
   k_mlp_features<0><<<nblocks(coarse->n, 256), 256, 0, stream>>>(
       coarse->n, reinterpret_cast<const float *>(coarse->pos),
-      reinterpret_cast<const float *>(coarse->vel), cs,
-      reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), mlp_const(c), feat,
-      flags, knn, nullptr, nullptr);
+      reinterpret_cast<const float *>(coarse->vel), cs, dev_grid(grid), mlp_const(c), feat, knn);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
@@ -857,20 +886,20 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   const float *pos = reinterpret_cast<const float *>(coarse->pos);
   const float *vel = reinterpret_cast<const float *>(coarse->vel);
   const DevGrid g = dev_grid(grid);
-  prof_begin(GG_STAGE_MLP_FEATURES, stream);
-  k_zero_int<<<1, 1, 0, stream>>>(fb_count);
-  GG_LAUNCH_CHECK();
-  k_mlp_features<1><<<nblocks(n, 256), 256, 0, stream>>>(
-      n, pos, vel, cs, reinterpret_cast<const float *>(boundary->pos), bcs, g, mlp_const(c), feat,
-      flags, nullptr, fb_list, fb_count);
-  GG_LAUNCH_CHECK();
-  prof_end(GG_STAGE_MLP_FEATURES, stream);
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const uint2 *feat4 = reinterpret_cast<const uint2 *>(feat);
-  // flagged particles: the teacher (the plain upsampler on the template), on
-  // the side stream while the network runs (the two write disjoint outputs)
+  // flags and the compacted fallback list first, so that the teacher on the
+  // flagged particles (the plain upsampler on the template) runs on the side
+  // stream beside the features and the network (they write disjoint outputs)
+  prof_begin(GG_STAGE_MLP_FEATURES, stream);
+  k_zero_int<<<1, 1, 0, stream>>>(fb_count);
+  GG_LAUNCH_CHECK();
+  k_mlp_flags<<<nblocks(n, 256), 256, 0, stream>>>(
+      n, pos, reinterpret_cast<const float *>(boundary->pos), bcs, g, mlp_const(c), flags,
+      fb_list, fb_count);
+  GG_LAUNCH_CHECK();
   SideStream *side = nullptr;
   GG_TRY(side_stream(side));
   if (hipEventRecord(side->fork, stream) != hipSuccess ||
@@ -883,6 +912,13 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   GG_LAUNCH_CHECK();
   prof_end(GG_STAGE_MLP_FALLBACK, side->st);
   if (hipEventRecord(side->join, side->st) != hipSuccess) return GG_ERR_HIP;
+  k_mlp_features<1><<<nblocks(n, 256), 256, 0, stream>>>(n, pos, vel, cs, g, mlp_const(c), feat,
+                                                         nullptr);
+  if (hipGetLastError() != hipSuccess) {
+    (void)hipStreamWaitEvent(stream, side->join, 0);
+    return GG_ERR_HIP;
+  }
+  prof_end(GG_STAGE_MLP_FEATURES, stream);
   prof_begin(GG_STAGE_MLP, stream);
   int rc;
 #define GG_INFER(W_, T_)                                                                     \

@@ -9,13 +9,17 @@
 
 namespace gg {
 namespace {
+constexpr int kMaxStages = 64;
 struct Prof {
   bool on = false;
   bool paused = false;
   std::vector<hipEvent_t> ev;
   std::vector<std::pair<int, int>> rec;  // (stage, first event index)
   int next = 0;
-  int open_stage = -1;
+  int open[kMaxStages];  // first event index of each stage's open bracket, or -1
+  Prof() {
+    for (int &o : open) o = -1;
+  }
 };
 Prof &prof() {
   static Prof p;
@@ -30,26 +34,26 @@ static bool capturing(hipStream_t st) {
   return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
 }
 
+// Each stage has its own open bracket, so brackets of different stages may
+// interleave (e.g. one on a side stream inside another on the main stream).
 void prof_begin(int stage, hipStream_t st) {
   Prof &p = prof();
-  if (!p.on || p.paused || p.next + 2 > (int)p.ev.size() || capturing(st)) {
-    p.open_stage = -1;
-    return;
-  }
-  if (hipEventRecord(p.ev[p.next], st) != hipSuccess) {
-    p.open_stage = -1;
-    return;
-  }
-  p.open_stage = stage;
+  if (stage < 0 || stage >= kMaxStages) return;
+  p.open[stage] = -1;
+  if (!p.on || p.paused || p.next + 2 > (int)p.ev.size() || capturing(st)) return;
+  if (hipEventRecord(p.ev[p.next], st) != hipSuccess) return;
+  p.open[stage] = p.next;
+  p.next += 2;
 }
 
 void prof_end(int stage, hipStream_t st) {
   Prof &p = prof();
-  if (!p.on || p.open_stage != stage) return;
-  if (hipEventRecord(p.ev[p.next + 1], st) != hipSuccess) return;
-  p.rec.emplace_back(stage, p.next);
-  p.next += 2;
-  p.open_stage = -1;
+  if (stage < 0 || stage >= kMaxStages) return;
+  const int first = p.open[stage];
+  p.open[stage] = -1;
+  if (!p.on || first < 0) return;
+  if (hipEventRecord(p.ev[first + 1], st) != hipSuccess) return;
+  p.rec.emplace_back(stage, first);
 }
 }  // namespace gg
 
@@ -63,6 +67,7 @@ extern "C" int gg_profiler_enable(int max_records) {
   p.next = 0;
   p.on = false;
   p.paused = false;
+  for (int &o : p.open) o = -1;
   if (max_records <= 0) return GG_OK;
   p.ev.resize(2 * (size_t)max_records);
   for (auto &e : p.ev)
