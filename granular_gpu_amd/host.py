@@ -71,6 +71,13 @@ def load() -> C.CDLL:
         "ggh_system_set_solver": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float,
                                             C.c_int]),
         "ggh_darray_selftest": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
+        "ggh_write_vtk": (C.c_int, [C.c_char_p, C.c_char_p, vp, C.c_int]),
+        "ggh_system_save_upsampled_vtk": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int]),
+        "ggh_system_save_mass_vtk": (C.c_int, [vp, C.c_char_p, C.c_int, ip]),
+        "ggh_checkpoint_write": (C.c_int, [C.c_char_p, vp, vp, vp, vp, C.c_int, vp, vp, C.c_int,
+                                           vp, vp, C.c_int]),
+        "ggh_system_save_checkpoint": (C.c_int, [vp, C.c_char_p]),
+        "ggh_checkpoint_read": (C.c_int, [C.c_char_p, C.c_int, C.c_int, vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -139,6 +146,45 @@ def darray_selftest(n: int) -> str:
     if rc != 0:
         raise HostError(f"DArray self-test (n={n}) failed: {msg}")
     return msg
+
+
+def write_vtk(path, points, title="Upsampled Granular Particles") -> int:
+    """The reference's legacy-ASCII VTK point file (NaN points dropped, y/z
+    swapped); returns the number of points written."""
+    lib = load()
+    p = _f3(points)
+    return _chk(lib, lib.ggh_write_vtk(str(path).encode(), title.encode(), p.ctypes.data,
+                                       p.shape[0]), "ggh_write_vtk")
+
+
+def write_checkpoint(path, pos, vel, mass, adaptive_last_step, boundary, boundary_animated,
+                     fine_pos, fine_vel):
+    lib = load()
+    p, v, b, fp, fv = _f3(pos), _f3(vel), _f3(boundary), _f3(fine_pos), _f3(fine_vel)
+    m = np.ascontiguousarray(mass, np.float32)
+    a = np.ascontiguousarray(adaptive_last_step, np.int32)
+    ba = np.ascontiguousarray(boundary_animated, np.int32)
+    _chk(lib, lib.ggh_checkpoint_write(str(path).encode(), p.ctypes.data, v.ctypes.data,
+                                       m.ctypes.data, a.ctypes.data, p.shape[0], b.ctypes.data,
+                                       ba.ctypes.data, b.shape[0], fp.ctypes.data,
+                                       fv.ctypes.data, fp.shape[0]), "ggh_checkpoint_write")
+
+
+def read_checkpoint(path) -> dict:
+    """All arrays of a checkpoint file, keyed (set, field)."""
+    lib = load()
+    out = {}
+    for which, field in (("granular", "pos"), ("granular", "vel"), ("granular", "mass"),
+                         ("granular", "adaptive_last_step"), ("boundary", "pos"),
+                         ("boundary", "is_animated"), ("fine", "pos"), ("fine", "vel")):
+        code, dt, w = FIELDS[field]
+        n = _chk(lib, lib.ggh_checkpoint_read(str(path).encode(), SETS[which], code, None, 0),
+                 "ggh_checkpoint_read")
+        a = np.zeros((n, w) if w > 1 else n, dt)
+        _chk(lib, lib.ggh_checkpoint_read(str(path).encode(), SETS[which], code, a.ctypes.data,
+                                          n), "ggh_checkpoint_read")
+        out[(which, field)] = a
+    return out
 
 
 class HostSystem:
@@ -217,6 +263,22 @@ class HostSystem:
     def save_stats(self, path):
         _chk(self.lib, self.lib.ggh_system_save_stats(self.h, str(path).encode()),
              "ggh_system_save_stats")
+
+    def save_upsampled_vtk(self, save_file, frame, is_adaptive=False) -> int:
+        return _chk(self.lib, self.lib.ggh_system_save_upsampled_vtk(
+            self.h, str(save_file).encode(), int(bool(is_adaptive)), int(frame)),
+            "ggh_system_save_upsampled_vtk")
+
+    def save_mass_vtk(self, save_file, frame) -> np.ndarray:
+        counts = np.zeros(3, np.int32)
+        _chk(self.lib, self.lib.ggh_system_save_mass_vtk(
+            self.h, str(save_file).encode(), int(frame),
+            counts.ctypes.data_as(C.POINTER(C.c_int))), "ggh_system_save_mass_vtk")
+        return counts
+
+    def save_checkpoint(self, path):
+        _chk(self.lib, self.lib.ggh_system_save_checkpoint(self.h, str(path).encode()),
+             "ggh_system_save_checkpoint")
 
     def set_solver(self, stab=3, solver=5, mu_s=0.0, mu_k=0.0, flags=0):
         _chk(self.lib, self.lib.ggh_system_set_solver(self.h, stab, solver, mu_s, mu_k, flags),

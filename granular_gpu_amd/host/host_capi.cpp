@@ -1,15 +1,17 @@
 // C-ABI over the host C++ API (handles for ctypes and other FFI callers).
 #include "granular_host.h"
 
+#include <algorithm>
 #include <cstring>
+#include <exception>
 #include <iostream>
 #include <sstream>
-#include <exception>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "granular/GranularSystem.hpp"
+#include "granular/Output.hpp"
 #include "granular/SceneConfig.hpp"
 
 struct ggh_system {
@@ -316,6 +318,102 @@ int ggh_system_set_solver(ggh_system *h, int stab, int solver, float mu_s, float
     s.set_friction(mu_s, mu_k);
     s.set_flags(flags);
     return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_write_vtk(const char *path, const char *title, const float *xyz, int n) {
+  try {
+    if (!path || !title || n < 0 || (n > 0 && !xyz)) throw std::invalid_argument("bad arguments");
+    return granular_output::write_vtk_points(path, title, triples(xyz, n));
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_save_upsampled_vtk(const ggh_system *h, const char *save_file, int is_adaptive,
+                                  int frame) {
+  try {
+    if (!h || !save_file) throw std::invalid_argument("bad arguments");
+    return granular_output::save_upsampled_vtk(*h->sys->get_upsampled(), save_file,
+                                               is_adaptive != 0, frame);
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_save_mass_vtk(const ggh_system *h, const char *save_file, int frame,
+                             int counts[3]) {
+  try {
+    if (!h || !save_file || !counts) throw std::invalid_argument("bad arguments");
+    granular_output::save_particles_by_mass_vtk(*h->sys->get_particles(), save_file, frame,
+                                                counts);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_checkpoint_write(const char *path, const float *pos, const float *vel,
+                         const float *mass, const int *adaptive_last_step, int n,
+                         const float *boundary, const int *boundary_animated, int nb,
+                         const float *fine_pos, const float *fine_vel, int nf) {
+  try {
+    if (!path || n < 0 || nb < 0 || nf < 0) throw std::invalid_argument("bad arguments");
+    if ((n > 0 && (!pos || !vel || !mass || !adaptive_last_step)) ||
+        (nb > 0 && (!boundary || !boundary_animated)) || (nf > 0 && (!fine_pos || !fine_vel)))
+      throw std::invalid_argument("null array");
+    granular_output::Checkpoint c;
+    c.pos = triples(pos, n);
+    c.vel = triples(vel, n);
+    c.mass.assign(mass, mass + n);
+    c.adaptive_last_step.assign(adaptive_last_step, adaptive_last_step + n);
+    c.boundary_pos = triples(boundary, nb);
+    c.boundary_animated.assign(boundary_animated, boundary_animated + nb);
+    c.fine_pos = triples(fine_pos, nf);
+    c.fine_vel = triples(fine_vel, nf);
+    granular_output::write_checkpoint(path, c);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_save_checkpoint(const ggh_system *h, const char *path) {
+  try {
+    if (!h || !path) throw std::invalid_argument("bad arguments");
+    granular_output::write_checkpoint(path, granular_output::checkpoint_of(*h->sys));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_checkpoint_read(const char *path, int which, int field, void *out, int max) {
+  try {
+    if (!path || max < 0) throw std::invalid_argument("bad arguments");
+    const granular_output::Checkpoint c = granular_output::read_checkpoint(path);
+    const void *src = nullptr;
+    size_t count = 0, esz = 4;
+    auto pick = [&](const void *p, size_t n, size_t e) {
+      src = p;
+      count = n;
+      esz = e;
+    };
+    if (which == 0 && field == 0) pick(c.pos.data(), c.pos.size(), 12);
+    else if (which == 0 && field == 1) pick(c.vel.data(), c.vel.size(), 12);
+    else if (which == 0 && field == 2) pick(c.mass.data(), c.mass.size(), 4);
+    else if (which == 0 && field == 6)
+      pick(c.adaptive_last_step.data(), c.adaptive_last_step.size(), 4);
+    else if (which == 1 && field == 0) pick(c.boundary_pos.data(), c.boundary_pos.size(), 12);
+    else if (which == 1 && field == 7)
+      pick(c.boundary_animated.data(), c.boundary_animated.size(), 4);
+    else if (which == 2 && field == 0) pick(c.fine_pos.data(), c.fine_pos.size(), 12);
+    else if (which == 2 && field == 1) pick(c.fine_vel.data(), c.fine_vel.size(), 12);
+    else throw std::invalid_argument("unknown set or field");
+    if (out && count > 0) std::memcpy(out, src, esz * std::min(count, (size_t)max));
+    return (int)count;
   } catch (const std::exception &e) {
     return fail(e);
   }

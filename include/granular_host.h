@@ -72,6 +72,32 @@ int ggh_system_frame_times(const ggh_system *h, float *out, int max);
 /* Rows (merged, split, N before) of the adaptive passes; returns the count. */
 int ggh_system_particle_stats(const ggh_system *h, int *out, int max_rows);
 int ggh_system_save_stats(ggh_system *h, const char *path);
+
+/* Output formats (include/granular/Output.hpp).  VTK: the reference's
+ * legacy-ASCII point files (main.cu:188-372), NaN points dropped, y and z
+ * swapped.  ggh_write_vtk (CPU only) writes n xyz points and returns the
+ * count written; the system versions write the fine set to
+ * <save_file>_adaptive|_normal/<frame>.vtk (returns the count) or the
+ * granular set by mass class to <save_file>_mass_light|_medium|_heavy/
+ * <frame>.vtk (counts[3]). */
+int ggh_write_vtk(const char *path, const char *title, const float *xyz, int n);
+int ggh_system_save_upsampled_vtk(const ggh_system *h, const char *save_file, int is_adaptive,
+                                  int frame);
+int ggh_system_save_mass_vtk(const ggh_system *h, const char *save_file, int frame,
+                             int counts[3]);
+/* Binary checkpoint of the particle state (no reference counterpart): the
+ * granular set's pos, vel, mass, adaptive_last_step, the boundary's pos and
+ * is_animated, the fine set's pos and vel.  ggh_checkpoint_write (CPU only)
+ * writes host arrays; ggh_checkpoint_read copies up to max elements of one
+ * array (which: 0 granular, 1 boundary, 2 fine; field numbers as
+ * ggh_system_read: 0 pos, 1 vel, 2 mass, 6 adaptive_last_step, 7
+ * is_animated) and returns its length. */
+int ggh_checkpoint_write(const char *path, const float *pos, const float *vel,
+                         const float *mass, const int *adaptive_last_step, int n,
+                         const float *boundary, const int *boundary_animated, int nb,
+                         const float *fine_pos, const float *fine_vel, int nf);
+int ggh_system_save_checkpoint(const ggh_system *h, const char *path);
+int ggh_checkpoint_read(const char *path, int which, int field, void *out, int max);
 int ggh_system_set_solver(ggh_system *h, int stab, int solver, float mu_s, float mu_k,
                           int flags);
 

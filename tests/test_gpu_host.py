@@ -138,3 +138,29 @@ def test_system_moving_boundary(gpu):
     assert np.array_equal(sysm.read("granular", "pos"), cp)
     assert np.array_equal(sysm.read("granular", "vel"), cv)
     sysm.close()
+
+
+def test_system_outputs(gpu, tmp_path):
+    # VTK files and the checkpoint carry the system's own arrays
+    from granular_gpu_amd import host
+    _log("outputs: VTK and checkpoint")
+    s = scenes.plain_box(12, 1.0)
+    fine = scenes.fine_cloud(s.pos, 4, 0.005, seed=3)
+    sysm = host.HostSystem.from_scene(s, fine=fine, is_adaptive=1)
+    sysm.step(5)
+    fp = sysm.read("fine", "pos")
+    n = sysm.save_upsampled_vtk(tmp_path / "out", 7, is_adaptive=True)
+    assert n == int(np.isfinite(fp).all(axis=1).sum())
+    lines = (tmp_path / "out_adaptive" / "7.vtk").read_text().splitlines()
+    assert lines[4] == f"POINTS {n} float"
+    xzy = np.array([l.split() for l in lines[5:5 + n]], np.float64)
+    ok = fp[np.isfinite(fp).all(axis=1)].astype(np.float64)
+    assert np.allclose(xzy, ok[:, [0, 2, 1]], rtol=1e-5, atol=1e-6)
+    mass = sysm.read("granular", "mass")
+    counts = sysm.save_mass_vtk(tmp_path / "out", 7)
+    assert counts.tolist() == [int((mass < 2).sum()), int(((mass >= 2) & (mass <= 3)).sum()),
+                               int((mass > 3).sum())]
+    sysm.save_checkpoint(tmp_path / "s.ggck")
+    ck = host.read_checkpoint(tmp_path / "s.ggck")
+    for which, field in ck:
+        assert ck[(which, field)].tobytes() == sysm.read(which, field).tobytes(), (which, field)

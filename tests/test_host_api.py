@@ -108,3 +108,53 @@ def test_upsampled_cloud_generator():
     want = (c[:, None, :] + off).reshape(-1, 3)
     assert np.array_equal(got, want)
     assert np.all(np.abs(got.reshape(2, per, 3) - c[:, None]) <= r)
+
+
+def test_vtk_writer_format(tmp_path):
+    # the reference's point files (main.cu:188-253): NaN points dropped,
+    # "x z y" with the stream's default float formatting, one vertex cell each
+    pts = np.float32([[0.1, 0.2, 0.3], [np.nan, 0.0, 0.0], [1.5, -2.25, 3e-7]])
+    assert host.write_vtk(tmp_path / "a.vtk", pts) == 2
+    lines = (tmp_path / "a.vtk").read_text().splitlines()
+    assert lines[:5] == ["# vtk DataFile Version 3.0", "Upsampled Granular Particles", "ASCII",
+                         "DATASET UNSTRUCTURED_GRID", "POINTS 2 float"]
+    assert lines[5] == "0.1 0.3 0.2"
+    assert lines[6] == "1.5 3e-07 -2.25"
+    assert lines[7:] == ["CELLS 2 4", "1 0", "1 1", "CELL_TYPES 2", "1", "1"]
+    # every point NaN: the file is created and left empty, as in the reference
+    assert host.write_vtk(tmp_path / "b.vtk", np.full((2, 3), np.nan, np.float32)) == 0
+    assert (tmp_path / "b.vtk").read_text() == ""
+    with pytest.raises(host.HostError):
+        host.write_vtk(tmp_path / "no_such_dir" / "c.vtk", pts)
+
+
+def test_checkpoint_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    n, nb, nf = 1000, 300, 8000
+    arrs = dict(pos=rng.normal(size=(n, 3)).astype(np.float32),
+                vel=rng.normal(size=(n, 3)).astype(np.float32),
+                mass=rng.uniform(0.5, 4, n).astype(np.float32),
+                adaptive_last_step=rng.integers(0, 2, n).astype(np.int32),
+                boundary=rng.normal(size=(nb, 3)).astype(np.float32),
+                boundary_animated=rng.integers(0, 2, nb).astype(np.int32),
+                fine_pos=rng.normal(size=(nf, 3)).astype(np.float32),
+                fine_vel=rng.normal(size=(nf, 3)).astype(np.float32))
+    arrs["pos"][5] = np.nan  # bit patterns travel as they are
+    path = tmp_path / "state.ggck"
+    host.write_checkpoint(path, **arrs)
+    got = host.read_checkpoint(path)
+    pairs = {("granular", "pos"): "pos", ("granular", "vel"): "vel",
+             ("granular", "mass"): "mass", ("granular", "adaptive_last_step"):
+             "adaptive_last_step", ("boundary", "pos"): "boundary",
+             ("boundary", "is_animated"): "boundary_animated", ("fine", "pos"): "fine_pos",
+             ("fine", "vel"): "fine_vel"}
+    for key, name in pairs.items():
+        assert got[key].tobytes() == arrs[name].tobytes(), key
+    raw = path.read_bytes()
+    assert raw[:4] == b"GGCK" and len(raw) == 4 + 4 + 3 * 8 + n * 32 + nb * 16 + nf * 24
+    (tmp_path / "short.ggck").write_bytes(raw[:-1])
+    (tmp_path / "magic.ggck").write_bytes(b"XXCK" + raw[4:])
+    (tmp_path / "long.ggck").write_bytes(raw + b"\0")
+    for bad in ("short", "magic", "long"):
+        with pytest.raises(host.HostError):
+            host.read_checkpoint(tmp_path / f"{bad}.ggck")
