@@ -33,6 +33,15 @@ class ggh_scene(C.Structure):
                 ("save_file", C.c_char * 256)]
 
 
+class ggh_animation(C.Structure):
+    _fields_ = [("is_animating", C.c_int), ("animation_time", C.c_float),
+                ("start_index", C.c_int), ("num_particles", C.c_int),
+                ("translation_direction", C.c_float * 3), ("translation_speed", C.c_float),
+                ("translation_complete", C.c_int), ("rotation_angle", C.c_float),
+                ("rotation_speed", C.c_float), ("rotation_center", C.c_float * 3),
+                ("translation_time", C.c_float), ("rotation_limit", C.c_float)]
+
+
 class HostError(RuntimeError):
     pass
 
@@ -57,6 +66,10 @@ def load() -> C.CDLL:
         "ggh_scene_lattice": (C.c_int, [S, vp, C.c_int]),
         "ggh_scene_walls": (C.c_int, [S, vp, C.c_int]),
         "ggh_upsampled_cloud": (C.c_int, [vp, C.c_int, C.c_float, C.c_int, vp, C.c_int]),
+        "ggh_animation_default": (None, [C.POINTER(ggh_animation)]),
+        "ggh_scene_boundary": (C.c_int, [S, vp, vp, C.c_int, C.POINTER(ggh_animation)]),
+        "ggh_system_animate": (C.c_int, [vp, fp, C.c_float, fp]),
+        "ggh_system_update_animation": (C.c_int, [vp, C.POINTER(ggh_animation), C.c_float]),
         "ggh_system_create": (vp, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, fp, C.c_float,
                                    C.c_float, fp, ip, C.c_float, C.c_float, C.c_int, C.c_int,
                                    C.c_int]),
@@ -73,6 +86,7 @@ def load() -> C.CDLL:
         "ggh_darray_selftest": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
         "ggh_write_vtk": (C.c_int, [C.c_char_p, C.c_char_p, vp, C.c_int]),
         "ggh_system_save_upsampled_vtk": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int]),
+        "ggh_system_save_boundary_vtk": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int]),
         "ggh_system_save_mass_vtk": (C.c_int, [vp, C.c_char_p, C.c_int, ip]),
         "ggh_checkpoint_write": (C.c_int, [C.c_char_p, vp, vp, vp, vp, C.c_int, vp, vp, C.c_int,
                                            vp, vp, C.c_int]),
@@ -131,6 +145,27 @@ def scene_lattice(s: ggh_scene) -> np.ndarray:
 
 def scene_walls(s: ggh_scene) -> np.ndarray:
     return _emit(load().ggh_scene_walls, C.byref(s))
+
+
+def animation_default() -> ggh_animation:
+    """The reference's AnimationState defaults (main.cu:100-111)."""
+    a = ggh_animation()
+    load().ggh_animation_default(C.byref(a))
+    return a
+
+
+def scene_boundary(s: ggh_scene):
+    """Boundary set of the scene: (positions (n, 3), animated markers (n,),
+    animation state).  EXCAVATOR / FUNNEL add the obj_file point cloud when
+    that file exists."""
+    lib = load()
+    n = _chk(lib, lib.ggh_scene_boundary(C.byref(s), None, None, 0, None), "ggh_scene_boundary")
+    pos = np.zeros((n, 3), np.float32)
+    animated = np.zeros(n, np.int32)
+    anim = ggh_animation()
+    _chk(lib, lib.ggh_scene_boundary(C.byref(s), pos.ctypes.data, animated.ctypes.data, n,
+                                     C.byref(anim)), "ggh_scene_boundary")
+    return pos, animated, anim
 
 
 def upsampled_cloud(coarse, radius: float, per: int = 25) -> np.ndarray:
@@ -248,6 +283,20 @@ class HostSystem:
         return _chk(self.lib, self.lib.ggh_system_translate_animated(self.h, *map(float, d)),
                     "ggh_system_translate_animated")
 
+    def animate(self, translation, angle_deg=0.0, center=(0.0, 0.0, 0.0)):
+        """animateParticles on the boundary: translate the animated particles,
+        then rotate them by angle_deg about z through center."""
+        t = (C.c_float * 3)(*[float(v) for v in translation])
+        c = (C.c_float * 3)(*[float(v) for v in center])
+        _chk(self.lib, self.lib.ggh_system_animate(self.h, t, float(angle_deg), c),
+             "ggh_system_animate")
+
+    def update_animation(self, anim: ggh_animation, dt: float) -> bool:
+        """One updateAnimation(dt) call advancing `anim`; True if the
+        boundary moved."""
+        return bool(_chk(self.lib, self.lib.ggh_system_update_animation(
+            self.h, C.byref(anim), float(dt)), "ggh_system_update_animation"))
+
     def frame_times(self) -> np.ndarray:
         n = self.lib.ggh_system_frame_times(self.h, None, 0)
         out = np.zeros(n, np.float32)
@@ -268,6 +317,13 @@ class HostSystem:
         return _chk(self.lib, self.lib.ggh_system_save_upsampled_vtk(
             self.h, str(save_file).encode(), int(bool(is_adaptive)), int(frame)),
             "ggh_system_save_upsampled_vtk")
+
+    def save_boundary_vtk(self, save_file, frame, centred=False) -> int:
+        """The animated boundary particles as a VTK file; centred uses the
+        fine set's centroid and lowest point."""
+        return _chk(self.lib, self.lib.ggh_system_save_boundary_vtk(
+            self.h, str(save_file).encode(), int(frame), int(bool(centred))),
+            "ggh_system_save_boundary_vtk")
 
     def save_mass_vtk(self, save_file, frame) -> np.ndarray:
         counts = np.zeros(3, np.int32)

@@ -120,6 +120,38 @@ void save_particles_by_mass_vtk(const GranularParticles &particles,
   }
 }
 
+int save_animated_boundary_vtk(const GranularParticles &boundary, const GranularParticles *fine,
+                               const std::string &save_file, int frame) {
+  const std::string dir = save_file + (fine ? "_boundary_adaptive" : "");
+  std::filesystem::create_directories(dir);
+  const std::string path = dir + "/" + std::to_string(frame) + ".vtk";
+  std::ofstream f(path);
+  if (!f.is_open()) throw std::runtime_error("cannot open " + path);
+  const std::vector<float3> pos = download(boundary.get_pos_ptr(), boundary.size());
+  const std::vector<int> marks =
+      download(const_cast<GranularParticles &>(boundary).get_is_animated_ptr(), boundary.size());
+  std::vector<float3> moving;
+  for (size_t i = 0; i < pos.size(); ++i)
+    if (marks[i] == 1) moving.push_back(pos[i]);
+  if (moving.empty()) return 0;
+  if (fine) {
+    std::vector<float3> valid;
+    for (const auto &p : download(fine->get_pos_ptr(), fine->size()))
+      if (finite_point(p)) valid.push_back(p);
+    float3 c = make_float3(0.0f, 0.0f, 0.0f);
+    for (const auto &p : valid) c = make_float3(c.x + p.x, c.y + p.y, c.z + p.z);
+    const float n = (float)valid.size();
+    c = make_float3(c.x / n, c.y / n, c.z / n);
+    float min_y = 1000000.f;
+    for (const auto &p : valid) min_y = std::fmin(min_y, p.y - c.y);
+    const float lift = min_y < 0.0f ? -min_y : 0.0f;
+    for (auto &p : moving) p = make_float3(p.x - c.x, p.y - c.y + lift, p.z - c.z);
+  }
+  write_vtk(f, "Animated Boundary Particles", moving);
+  if (!f) throw std::runtime_error("write failed: " + path);
+  return (int)moving.size();
+}
+
 void write_checkpoint(const std::string &path, const Checkpoint &c) {
   const uint64_t n = c.pos.size(), nb = c.boundary_pos.size(), nf = c.fine_pos.size();
   if (c.vel.size() != n || c.mass.size() != n || c.adaptive_last_step.size() != n ||

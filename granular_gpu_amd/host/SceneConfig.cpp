@@ -4,6 +4,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstdlib>
+#include <filesystem>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -240,10 +241,8 @@ std::vector<float3> granular_lattice(const SceneConfig &c) {
 
 std::vector<float3> boundary_walls(const SceneConfig &c) {
   const int sc = c.scene;
-  if (sc == EXCAVATOR || sc == FUNNEL)
-    throw std::runtime_error("boundary_walls: the EXCAVATOR and FUNNEL scenes need external geometry");
-  if (sc != PILING && sc != BOX && sc != CORNER && sc != BOX_OPENING)
-    throw std::runtime_error("boundary_walls: unknown scene id");
+  if (sc < PILING || sc > BOX_OPENING) throw std::runtime_error("boundary_walls: unknown scene id");
+  const bool open_top = sc == EXCAVATOR || sc == FUNNEL;
   const float3 S = c.space_size;
   const int nx = 2 * (int)std::ceil(S.x / c.cell_length);
   const int ny = 2 * (int)std::ceil(S.y / c.cell_length);
@@ -266,7 +265,7 @@ std::vector<float3> boundary_walls(const SceneConfig &c) {
   for (int i = 0; i < nx; ++i)
     for (int j = 0; j < nz - 2; ++j) {
       put(i, 0, j + 1);
-      if (far) put(i, ny - 1, j + 1);
+      if (far && !open_top) put(i, ny - 1, j + 1);
     }
   if (sides)
     for (int i = 0; i < ny - 2; ++i)
@@ -274,9 +273,53 @@ std::vector<float3> boundary_walls(const SceneConfig &c) {
         put(0, i + 1, j + 1);
         if (far) put(nx - 1, i + 1, j + 1);
       }
+  if (open_top) return pos;
   const float3 t = c.boundary_translation;
   for (auto &p : pos) p = make_float3(p.x + t.x, p.y + t.y, p.z + t.z);
   return pos;
+}
+
+namespace {
+
+// Sum in insertion order, then divide by the count, in float.
+float3 centroid_of(const std::vector<float3> &v) {
+  float3 s = make_float3(0.0f, 0.0f, 0.0f);
+  for (const auto &p : v) s = make_float3(s.x + p.x, s.y + p.y, s.z + p.z);
+  const float n = (float)v.size();
+  return make_float3(s.x / n, s.y / n, s.z / n);
+}
+
+}  // namespace
+
+SceneBoundary scene_boundary(const SceneConfig &c) {
+  SceneBoundary b;
+  b.pos = boundary_walls(c);
+  b.animated.assign(b.pos.size(), 0);
+  if (c.scene != EXCAVATOR && c.scene != FUNNEL) return b;
+  if (c.obj_file.empty() || !std::filesystem::exists(c.obj_file)) return b;
+  int count = 0;
+  std::vector<float3> cloud = readBoundaryParticlesFromFile(c.obj_file, count);
+  const float3 centre = centroid_of(cloud);
+  b.anim.start_index = (int)b.pos.size();
+  b.anim.num_particles = (int)cloud.size();
+  b.anim.is_animating = true;
+  b.anim.animation_time = 0.0f;
+  if (c.scene == EXCAVATOR) {
+    b.anim.rotation_center = centre;
+    for (auto &p : cloud) p = make_float3(p.x + 2.5f, p.y + 0.0f, p.z + 0.5f);
+  } else {
+    const float angle = (float)(90.0 * M_PI / 180.0);
+    const float cs = std::cos(angle), sn = std::sin(angle);
+    for (auto &p : cloud) {
+      const float x = p.x - centre.x, y = p.y - centre.y, z = p.z - centre.z;
+      const float ry = y * cs - z * sn;
+      const float rz = y * sn + z * cs;
+      p = make_float3((x + centre.x) + 1.25f, (ry + centre.y) + 0.3f, (rz + centre.z) + 1.5f);
+    }
+  }
+  b.pos.insert(b.pos.end(), cloud.begin(), cloud.end());
+  b.animated.resize(b.pos.size(), 1);
+  return b;
 }
 
 std::vector<float3> upsampled_cloud(const std::vector<float3> &coarse, float radius, int per) {

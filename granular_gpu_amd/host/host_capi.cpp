@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "granular/Animation.hpp"
 #include "granular/GranularSystem.hpp"
 #include "granular/Output.hpp"
 #include "granular/SceneConfig.hpp"
@@ -87,6 +88,44 @@ SceneConfig from_c(const ggh_scene *s) {
   return c;
 }
 
+void to_c(const AnimationState &a, ggh_animation *o) {
+  o->is_animating = a.is_animating ? 1 : 0;
+  o->animation_time = a.animation_time;
+  o->start_index = a.start_index;
+  o->num_particles = a.num_particles;
+  o->translation_direction[0] = a.translation_direction.x;
+  o->translation_direction[1] = a.translation_direction.y;
+  o->translation_direction[2] = a.translation_direction.z;
+  o->translation_speed = a.translation_speed;
+  o->translation_complete = a.translation_complete ? 1 : 0;
+  o->rotation_angle = a.rotation_angle;
+  o->rotation_speed = a.rotation_speed;
+  o->rotation_center[0] = a.rotation_center.x;
+  o->rotation_center[1] = a.rotation_center.y;
+  o->rotation_center[2] = a.rotation_center.z;
+  o->translation_time = a.translation_time;
+  o->rotation_limit = a.rotation_limit;
+}
+
+AnimationState from_c(const ggh_animation *o) {
+  AnimationState a;
+  a.is_animating = o->is_animating != 0;
+  a.animation_time = o->animation_time;
+  a.start_index = o->start_index;
+  a.num_particles = o->num_particles;
+  a.translation_direction = make_float3(o->translation_direction[0], o->translation_direction[1],
+                                        o->translation_direction[2]);
+  a.translation_speed = o->translation_speed;
+  a.translation_complete = o->translation_complete != 0;
+  a.rotation_angle = o->rotation_angle;
+  a.rotation_speed = o->rotation_speed;
+  a.rotation_center =
+      make_float3(o->rotation_center[0], o->rotation_center[1], o->rotation_center[2]);
+  a.translation_time = o->translation_time;
+  a.rotation_limit = o->rotation_limit;
+  return a;
+}
+
 int emit(const std::vector<float3> &v, float *out, int max) {
   if (out) {
     const size_t k = std::min(v.size(), (size_t)(max > 0 ? max : 0));
@@ -149,6 +188,27 @@ int ggh_scene_walls(const ggh_scene *s, float *out, int max) {
   try {
     if (!s) throw std::invalid_argument("null scene");
     return emit(boundary_walls(from_c(s)), out, max);
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+void ggh_animation_default(ggh_animation *a) {
+  if (a) to_c(AnimationState{}, a);
+}
+
+int ggh_scene_boundary(const ggh_scene *s, float *out, int *animated, int max,
+                       ggh_animation *anim) {
+  try {
+    if (!s) throw std::invalid_argument("null scene");
+    const SceneBoundary b = scene_boundary(from_c(s));
+    emit(b.pos, out, max);
+    if (animated) {
+      const size_t k = std::min(b.animated.size(), (size_t)(max > 0 ? max : 0));
+      std::copy(b.animated.begin(), b.animated.begin() + (long)k, animated);
+    }
+    if (anim) to_c(b.anim, anim);
+    return (int)b.pos.size();
   } catch (const std::exception &e) {
     return fail(e);
   }
@@ -258,22 +318,40 @@ int ggh_system_translate_animated(ggh_system *h, float dx, float dy, float dz) {
     if (!h) throw std::invalid_argument("null handle");
     auto b = h->sys->get_boundaries();
     const unsigned int nb = b->size();
-    std::vector<float3> p(nb);
-    std::vector<int> a(nb);
     if (nb == 0) return 0;
-    if (hipMemcpy(p.data(), b->get_pos_ptr(), 12 * (size_t)nb, hipMemcpyDeviceToHost) != hipSuccess ||
+    std::vector<int> a(nb);
+    if (hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(a.data(), b->get_is_animated_ptr(), 4 * (size_t)nb, hipMemcpyDeviceToHost) !=
             hipSuccess)
       throw std::runtime_error("copy failed");
-    int moved = 0;
-    for (unsigned int i = 0; i < nb; ++i)
-      if (a[i] == 1) {
-        p[i] = make_float3(p[i].x + dx, p[i].y + dy, p[i].z + dz);
-        ++moved;
-      }
-    if (hipMemcpy(b->get_pos_ptr(), p.data(), 12 * (size_t)nb, hipMemcpyHostToDevice) != hipSuccess)
-      throw std::runtime_error("copy failed");
-    return moved;
+    granular_animation::animate_boundary(*h->sys, make_float3(dx, dy, dz), 0.0f,
+                                         make_float3(0.0f, 0.0f, 0.0f));
+    return (int)std::count(a.begin(), a.end(), 1);
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_animate(ggh_system *h, const float translation[3], float angle_deg,
+                       const float center[3]) {
+  try {
+    if (!h || !translation || !center) throw std::invalid_argument("null argument");
+    granular_animation::animate_boundary(*h->sys,
+                                         make_float3(translation[0], translation[1], translation[2]),
+                                         angle_deg, make_float3(center[0], center[1], center[2]));
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_update_animation(ggh_system *h, ggh_animation *a, float dt) {
+  try {
+    if (!h || !a) throw std::invalid_argument("null argument");
+    AnimationState s = from_c(a);
+    const bool moved = granular_animation::update_animation(*h->sys, s, dt);
+    to_c(s, a);
+    return moved ? 1 : 0;
   } catch (const std::exception &e) {
     return fail(e);
   }
@@ -338,6 +416,19 @@ int ggh_system_save_upsampled_vtk(const ggh_system *h, const char *save_file, in
     if (!h || !save_file) throw std::invalid_argument("bad arguments");
     return granular_output::save_upsampled_vtk(*h->sys->get_upsampled(), save_file,
                                                is_adaptive != 0, frame);
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_save_boundary_vtk(const ggh_system *h, const char *save_file, int frame,
+                                 int centred) {
+  try {
+    if (!h || !save_file) throw std::invalid_argument("bad arguments");
+    const auto fine = h->sys->get_upsampled();
+    return granular_output::save_animated_boundary_vtk(*h->sys->get_boundaries(),
+                                                       centred ? fine.get() : nullptr,
+                                                       save_file, frame);
   } catch (const std::exception &e) {
     return fail(e);
   }

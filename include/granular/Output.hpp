@@ -41,6 +41,17 @@ int save_upsampled_vtk(const GranularParticles &fine, const std::string &save_fi
 void save_particles_by_mass_vtk(const GranularParticles &particles,
                                 const std::string &save_file, int frame, int out[3]);
 
+// The animated (is_animated == 1) boundary particles, title "Animated
+// Boundary Particles", NaN points kept.  fine == nullptr:
+// saveFunnelBoundaryParticlesToVTK (main.cu:370-432), <save_file>/<frame>.vtk
+// with the positions as they are.  Otherwise saveBoundaryParticlesToVTK
+// (main.cu:434-525), <save_file>_boundary_adaptive/<frame>.vtk with the
+// positions centred on the centroid of the finite fine particles and lifted
+// by the same amount as the fine set's lowest point.  With no animated
+// particle the file is created empty.  Returns the count written.
+int save_animated_boundary_vtk(const GranularParticles &boundary, const GranularParticles *fine,
+                               const std::string &save_file, int frame);
+
 struct Checkpoint {
   std::vector<float3> pos, vel;  // granular set
   std::vector<float> mass;

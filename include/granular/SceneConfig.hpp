@@ -46,11 +46,47 @@ std::vector<float3> readBoundaryParticlesFromFile(const std::string &filename, i
 // particle_translation with s = init_spacing, loops i (x count), j, k.
 std::vector<float3> granular_lattice(const SceneConfig &c);
 
-// Wall particles of the PILING, BOX, CORNER and BOX_OPENING recipes
-// (main.cu:822-1167) in the reference's insertion order, translated by
-// boundary_translation.  EXCAVATOR and FUNNEL need external geometry and
-// throw std::runtime_error.
+// Wall particles of the six recipes (main.cu:822-1167) in the reference's
+// insertion order.  PILING, BOX, CORNER and BOX_OPENING are translated by
+// boundary_translation; EXCAVATOR and FUNNEL build the box without its top
+// (y = max) wall and without the translation (main.cu:889-925, 985-1021).
 std::vector<float3> boundary_walls(const SceneConfig &c);
+
+// Kinematic motion of the animated boundary particles.  The fields and their
+// defaults are the reference's AnimationState (main.cu:100-111);
+// translation_time and rotation_limit are the reference's hard-coded 2.5 s
+// and 45 degrees of updateAnimation (main.cu:1219, 1255-1262), made
+// settable for scripted moves.
+struct AnimationState {
+  bool is_animating = false;
+  float animation_time = 0.0f;
+  int start_index = 0;
+  int num_particles = 0;
+  float3 translation_direction = make_float3(-0.1f, 0.0f, 0.0f);
+  float translation_speed = 5.0f;
+  bool translation_complete = false;
+  float rotation_angle = 0.0f;
+  float rotation_speed = 90.0f;
+  float3 rotation_center = make_float3(0.0f, 0.0f, 0.0f);
+  float translation_time = 2.5f;
+  float rotation_limit = 45.0f;
+};
+
+// The boundary set of a scene: walls, then (EXCAVATOR, FUNNEL) the point
+// cloud of obj_file if that file exists (main.cu:927-968, 1023-1075):
+//  - EXCAVATOR: cloud + (2.5, 0, 0.5); rotation_center = centroid of the
+//    cloud as read;
+//  - FUNNEL: cloud rotated 90 degrees about the x axis through its
+//    centroid (cosf/sinf of the float-rounded angle), then + (1.25, 0.3, 1.5).
+// Cloud particles are marked animated (1), walls 0; `anim` receives
+// start_index = wall count, num_particles = cloud size, is_animating = true.
+// A missing obj_file leaves walls only (the reference skips it silently).
+struct SceneBoundary {
+  std::vector<float3> pos;
+  std::vector<int> animated;
+  AnimationState anim;
+};
+SceneBoundary scene_boundary(const SceneConfig &c);
 
 // `per` fine particles per coarse particle, offsets drawn per axis from
 // std::uniform_real_distribution<float>(-radius, radius) on one

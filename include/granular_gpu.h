@@ -156,6 +156,18 @@ int gg_grid_build(gg_particles *set, gg_adaptive *ad, int *cell_start,
 int gg_boundary_build(gg_particles *boundary, int *bcell_start,
                       const gg_workspace *ws, hipStream_t stream);
 
+/* Kinematic boundary motion.  Every particle i with is_animated[i] == 1
+ * moves by `translation`; then, if angle_deg != 0, it is rotated by
+ * angle_deg degrees about the z axis through (center.x, center.y):
+ *   u = x - cx, v = y - cy, x' = (u c - v s) + cx, y' = (u s + v c) + cy,
+ * with c, s = cosf, sinf of (float)(angle_deg * pi / 180) evaluated once on
+ * the host.  Other particles are untouched.  Call gg_boundary_build before
+ * the next step (the moving-boundary re-sort).
+ * Replaces: animateParticles (main.cu:1175-1208), which evaluates cosf/sinf
+ * per thread on the device. */
+int gg_boundary_animate(gg_particles *boundary, gg_float3 translation, float angle_deg,
+                        gg_float3 center, hipStream_t stream);
+
 /* ---- plain PBD step ----------------------------------------------------- */
 /* Grid build + predict + stab/solver contact sweeps + finalize for the
  * granular set, against a boundary prepared by gg_boundary_build.

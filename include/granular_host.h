@@ -6,8 +6,8 @@
  *
  * Reference interface replaced: the GranularSystem class and its accessors
  * (GranularSystem.hpp:8-56), Solver::save_particle_stats (Solver.cu:1444),
- * loadSceneConfig (main.cu:123-167) and the scene builders of main.cu
- * (747-816, 822-1167). */
+ * loadSceneConfig (main.cu:123-167), the scene builders of main.cu
+ * (747-816, 822-1167) and its boundary animation (1175-1277). */
 #ifndef GRANULAR_HOST_H
 #define GRANULAR_HOST_H
 
@@ -47,6 +47,36 @@ int ggh_scene_lattice(const ggh_scene *s, float *out, int max);
 int ggh_scene_walls(const ggh_scene *s, float *out, int max);
 int ggh_upsampled_cloud(const float *coarse, int n, float radius, int per, float *out, int max);
 
+/* Kinematic animation state.  Fields and defaults of the reference's
+ * AnimationState (main.cu:100-111); translation_time (2.5 s) and
+ * rotation_limit (45 degrees) are constants of its updateAnimation
+ * (main.cu:1219, 1255-1262). */
+typedef struct ggh_animation {
+  int is_animating;
+  float animation_time;
+  int start_index;
+  int num_particles;
+  float translation_direction[3];
+  float translation_speed;
+  int translation_complete;
+  float rotation_angle;
+  float rotation_speed;
+  float rotation_center[3];
+  float translation_time;
+  float rotation_limit;
+} ggh_animation;
+void ggh_animation_default(ggh_animation *a);
+
+/* Boundary set of a scene (CPU only; main.cu:822-1167): the walls, then for
+ * EXCAVATOR / FUNNEL the xyz point cloud of obj_file when that file exists,
+ * transformed as the reference does (bucket + (2.5, 0, 0.5); funnel rotated
+ * 90 degrees about x through its centroid, + (1.25, 0.3, 1.5)).  Writes up
+ * to max xyz triples to out and animated markers (0 walls, 1 cloud) to
+ * animated (either may be NULL), fills *anim if not NULL, and returns the
+ * total count. */
+int ggh_scene_boundary(const ggh_scene *s, float *out, int *animated, int max,
+                       ggh_animation *anim);
+
 /* System: host arrays are copied to the device; the constructor runs one
  * step (as the reference's does). */
 ggh_system *ggh_system_create(const float *pos, int n, const float *boundary, int nb,
@@ -66,8 +96,17 @@ int ggh_system_size(const ggh_system *h, int which);
  * 6 adaptive_last_step (int), 7 is_animated (int). */
 int ggh_system_read(const ggh_system *h, int which, int field, void *out, int count);
 /* Moves the boundary particles flagged as animated by (dx,dy,dz) (a
- * kinematic boundary; the next step re-sorts a moving boundary). */
+ * kinematic boundary; the next step re-sorts a moving boundary); returns
+ * the number of animated particles. */
 int ggh_system_translate_animated(ggh_system *h, float dx, float dy, float dz);
+/* animateParticles (main.cu:1175-1208) on the boundary set: translation,
+ * then a rotation by angle_deg about the z axis through center (x, y);
+ * see gg_boundary_animate. */
+int ggh_system_animate(ggh_system *h, const float translation[3], float angle_deg,
+                       const float center[3]);
+/* One updateAnimation(dt) call (main.cu:1210-1277) advancing *a; returns 1
+ * if the boundary moved, 0 if not, negative on error. */
+int ggh_system_update_animation(ggh_system *h, ggh_animation *a, float dt);
 int ggh_system_frame_times(const ggh_system *h, float *out, int max);
 /* Rows (merged, split, N before) of the adaptive passes; returns the count. */
 int ggh_system_particle_stats(const ggh_system *h, int *out, int max_rows);
@@ -85,6 +124,13 @@ int ggh_system_save_upsampled_vtk(const ggh_system *h, const char *save_file, in
                                   int frame);
 int ggh_system_save_mass_vtk(const ggh_system *h, const char *save_file, int frame,
                              int counts[3]);
+/* The animated boundary particles: centred == 0 writes them as they are to
+ * <save_file>/<frame>.vtk (saveFunnelBoundaryParticlesToVTK, main.cu:370);
+ * centred != 0 centres and lifts them by the fine set's centroid and lowest
+ * point into <save_file>_boundary_adaptive/<frame>.vtk
+ * (saveBoundaryParticlesToVTK, main.cu:434).  Returns the count. */
+int ggh_system_save_boundary_vtk(const ggh_system *h, const char *save_file, int frame,
+                                 int centred);
 /* Binary checkpoint of the particle state (no reference counterpart): the
  * granular set's pos, vel, mass, adaptive_last_step, the boundary's pos and
  * is_animated, the fine set's pos and vel.  ggh_checkpoint_write (CPU only)

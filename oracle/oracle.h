@@ -164,6 +164,31 @@ float oracle_radius_surface(float m, float density);
 float oracle_radius_split(float m, float density);
 float oracle_expf(float x);
 
+/* ---- kinematic boundary (oracle_motion.cpp) ------------------------------ */
+/* Same layout as ggh_animation.                                           */
+typedef struct oracle_animation {
+  int is_animating;
+  float animation_time;
+  int start_index;
+  int num_particles;
+  float translation_direction[3];
+  float translation_speed;
+  int translation_complete;
+  float rotation_angle;
+  float rotation_speed;
+  float rotation_center[3];
+  float translation_time;
+  float rotation_limit;
+} oracle_animation;
+void oracle_rotation_cs(float angle_deg, float *c, float *s);
+/* animateParticles on n AoS positions: marked (== 1) particles translate,
+ * then rotate by angle_deg about z through (cx, cy) if angle_deg != 0.    */
+void oracle_animate(int n, float *pos, const int *animated, float tx, float ty, float tz,
+                    float angle_deg, float cx, float cy);
+/* One updateAnimation(dt) call; returns 1 if the positions moved.         */
+int oracle_update_animation(oracle_animation *a, int n, float *pos, const int *animated,
+                            float dt);
+
 /* Number of OpenMP threads the oracle uses (0 = default).                */
 void oracle_set_threads(int t);
 int oracle_get_threads(void);

@@ -34,6 +34,17 @@ class oracle_upsample_params(C.Structure):
         "floor_friction")]
 
 
+class oracle_animation(C.Structure):
+    _fields_ = [
+        ("is_animating", C.c_int), ("animation_time", C.c_float), ("start_index", C.c_int),
+        ("num_particles", C.c_int), ("translation_direction", C.c_float * 3),
+        ("translation_speed", C.c_float), ("translation_complete", C.c_int),
+        ("rotation_angle", C.c_float), ("rotation_speed", C.c_float),
+        ("rotation_center", C.c_float * 3), ("translation_time", C.c_float),
+        ("rotation_limit", C.c_float),
+    ]
+
+
 _lib = None
 
 
@@ -85,6 +96,13 @@ def load() -> C.CDLL:
         getattr(lib, nm).restype = C.c_float
     lib.oracle_expf.argtypes = [C.c_float]
     lib.oracle_expf.restype = C.c_float
+    lib.oracle_rotation_cs.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.oracle_rotation_cs.restype = None
+    lib.oracle_animate.argtypes = [C.c_int, fp, ip, C.c_float, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, C.c_float]
+    lib.oracle_animate.restype = None
+    lib.oracle_update_animation.argtypes = [C.POINTER(oracle_animation), C.c_int, fp, ip,
+                                            C.c_float]
     lib.oracle_set_threads.argtypes = [C.c_int]
     lib.oracle_get_threads.restype = C.c_int
     _lib = lib
@@ -364,3 +382,44 @@ def bf16_round(x):
     lib = load()
     _mlp_protos(lib)
     return np.float32(lib.oracle_bf16_round(float(x)))
+
+
+def animation_state(**kw) -> oracle_animation:
+    """An animation state with the reference's AnimationState defaults
+    (main.cu:100-111) and its 2.5 s / 45 degree limits, overridden by kw."""
+    a = oracle_animation()
+    a.translation_direction[:] = (-0.1, 0.0, 0.0)
+    a.translation_speed = 5.0
+    a.rotation_speed = 90.0
+    a.translation_time = 2.5
+    a.rotation_limit = 45.0
+    for k, v in kw.items():
+        if k in ("translation_direction", "rotation_center"):
+            getattr(a, k)[:] = tuple(float(x) for x in v)
+        else:
+            setattr(a, k, v)
+    return a
+
+
+def rotation_cs(angle_deg):
+    c, s = C.c_float(), C.c_float()
+    load().oracle_rotation_cs(float(angle_deg), C.byref(c), C.byref(s))
+    return np.float32(c.value), np.float32(s.value)
+
+
+def animate(pos, animated, translation, angle_deg=0.0, center=(0.0, 0.0)):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    animated = np.ascontiguousarray(animated, np.int32)
+    load().oracle_animate(pos.shape[0], pos.reshape(-1), animated,
+                          *(float(v) for v in translation), float(angle_deg),
+                          float(center[0]), float(center[1]))
+    return pos
+
+
+def update_animation(state: oracle_animation, pos, animated, dt):
+    """One updateAnimation(dt) call on pos (modified in place); returns
+    True if the positions moved."""
+    assert pos.dtype == np.float32 and pos.flags.c_contiguous
+    return bool(load().oracle_update_animation(
+        C.byref(state), pos.shape[0], pos.reshape(-1),
+        np.ascontiguousarray(animated, np.int32), float(dt)))

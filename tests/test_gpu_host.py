@@ -140,6 +140,50 @@ def test_system_moving_boundary(gpu):
     sysm.close()
 
 
+def test_boundary_animation_matches_oracle(gpu, tmp_path):
+    # animateParticles and the updateAnimation timeline on the device against
+    # the oracle on the same arrays, bit for bit; then the moving-boundary
+    # re-sort of the next step
+    from granular_gpu_amd import host
+    _log("boundary animation: gg_boundary_animate + update_animation")
+    s = scenes.plain_box(8, 1.0, dt=0.002, g=-9.8)
+    anim = np.zeros(s.boundary.shape[0], np.int32)
+    anim[s.boundary[:, 2] < 0.02] = 1            # the back wall
+    sysm = host.HostSystem.from_scene(s, is_move_boundary=True, boundary_animated=anim)
+    b = sysm.read("boundary", "pos")
+    a = sysm.read("boundary", "is_animated")
+    assert int(a.sum()) == int(anim.sum())
+    sysm.animate((0.01, 0.0, -0.02), -33.0, (0.5, 0.4, 0.0))
+    b = po.animate(b, a, (0.01, 0.0, -0.02), -33.0, (0.5, 0.4))
+    assert np.array_equal(sysm.read("boundary", "pos"), b)
+    st = host.animation_default()
+    st.is_animating, st.start_index, st.num_particles = 1, 40, 300
+    ost = po.animation_state(is_animating=1, start_index=40, num_particles=300)
+    dt = 0.05
+    calls = 0
+    while ost.is_animating:
+        assert sysm.update_animation(st, dt) == po.update_animation(ost, b, a, dt)
+        calls += 1
+        assert calls < 100
+    assert st.is_animating == 0 and st.translation_complete == 1
+    assert st.rotation_angle == ost.rotation_angle
+    assert list(st.rotation_center) == list(ost.rotation_center)
+    assert np.array_equal(sysm.read("boundary", "pos"), b)
+    sysm.step(1)
+    bpos, _ = po.sorted_boundary(b, s.cell_length, s.cells)
+    assert np.array_equal(sysm.read("boundary", "pos"), bpos)
+    assert np.isfinite(sysm.read("granular", "pos")).all()
+    # the animated particles as a VTK file, in set order, y and z swapped
+    marks = sysm.read("boundary", "is_animated")
+    n = sysm.save_boundary_vtk(tmp_path / "bnd", 3)
+    assert n == int((marks == 1).sum())
+    lines = (tmp_path / "bnd" / "3.vtk").read_text().splitlines()
+    assert lines[1] == "Animated Boundary Particles" and lines[4] == f"POINTS {n} float"
+    xzy = np.array([l.split() for l in lines[5:5 + n]], np.float64)
+    assert np.allclose(xzy, bpos[marks == 1][:, [0, 2, 1]], rtol=1e-5, atol=1e-6)
+    sysm.close()
+
+
 def test_system_outputs(gpu, tmp_path):
     # VTK files and the checkpoint carry the system's own arrays
     from granular_gpu_amd import host

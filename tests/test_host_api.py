@@ -71,10 +71,118 @@ def test_walls_match_python_recipe(name, recipe):
 
 
 @pytest.mark.parametrize("name", ["excavator", "funnel"])
-def test_walls_need_geometry(name):
+def test_open_top_walls(name):
+    # box without its top wall, never translated (main.cu:889-925, 985-1021)
     s = host.load_scene(Path(__file__).parent / "golden" / "scenes" / f"{name}.json")
-    with pytest.raises(host.HostError, match="external geometry"):
-        host.scene_walls(s)
+    w = host.scene_walls(s)
+    assert np.array_equal(w, scenes.box_walls(tuple(s.space_size), s.cell_length, name))
+    nx, ny, nz = (2 * c for c in scenes.cell_counts(tuple(s.space_size), s.cell_length))
+    assert w.shape[0] == 2 * nx * ny + nx * (nz - 2) + 2 * (ny - 2) * (nz - 2)
+    S = np.asarray(s.space_size, np.float32)
+    lo, hi = f32(0.005) * S, f32(0.99) * S + f32(0.005) * S
+    # at the top height only the upper edges of the front and back walls
+    at_top = np.isclose(w[:, 1], hi[1], atol=1e-6)
+    assert at_top.sum() == 2 * nx
+    assert (np.isclose(w[at_top, 2], lo[2], atol=1e-6) |
+            np.isclose(w[at_top, 2], hi[2], atol=1e-6)).all()
+    assert np.isclose(w[:, 1], lo[1], atol=1e-6).sum() == nx * nz
+    # without a point-cloud file the boundary is the walls alone
+    pos, animated, anim = host.scene_boundary(s)
+    assert np.array_equal(pos, w) and not animated.any() and anim.is_animating == 0
+
+
+def _scene_with_cloud(name, path):
+    s = host.load_scene(Path(__file__).parent / "golden" / "scenes" / f"{name}.json")
+    s.obj_file = str(path).encode()
+    return s
+
+
+def _cloud(n=257, seed=3):
+    rng = np.random.default_rng(seed)
+    return (rng.random((n, 3)) * [0.6, 0.7, 0.4] - [0.3, 0.0, 0.2]).astype(np.float32)
+
+
+def _float_centroid(c):
+    """Insertion-order float32 sum, then division by float(n)."""
+    acc = np.zeros(3, np.float32)
+    for p in c:
+        acc = acc + p
+    return acc / f32(len(c))
+
+
+def test_point_cloud_reader(tmp_path):
+    # whitespace-separated triples; reading stops at the first incomplete one
+    f = tmp_path / "c.txt"
+    f.write_text("0.5 0.25 -1\n  2e-3\t4 5\n\n6 7 8 9 10\n")
+    s = _scene_with_cloud("excavator", f)
+    pos, animated, anim = host.scene_boundary(s)
+    nw = host.scene_walls(s).shape[0]
+    assert pos.shape[0] == nw + 3 and anim.num_particles == 3
+    want = np.float32([[0.5, 0.25, -1], [2e-3, 4, 5], [6, 7, 8]])
+    assert np.array_equal(pos[nw:], want + np.float32([2.5, 0.0, 0.5]))
+
+
+def test_excavator_bucket(tmp_path):
+    c = _cloud()
+    f = tmp_path / "bucket.txt"
+    f.write_text("".join(f"{x!r} {y!r} {z!r}\n" for x, y, z in c.tolist()))
+    s = _scene_with_cloud("excavator", f)
+    pos, animated, anim = host.scene_boundary(s)
+    w = host.scene_walls(s)
+    nw = w.shape[0]
+    assert np.array_equal(pos[:nw], w)
+    # cloud shifted by (2.5, 0, 0.5) in float32, marked animated
+    assert np.array_equal(pos[nw:], c + np.float32([2.5, 0.0, 0.5]))
+    assert not animated[:nw].any() and (animated[nw:] == 1).all()
+    assert (anim.is_animating, anim.start_index, anim.num_particles) == (1, nw, len(c))
+    assert anim.translation_complete == 0 and anim.animation_time == 0
+    # rotation centre: centroid of the cloud as read, before the shift
+    g = np.array(anim.rotation_center, np.float32)
+    assert np.array_equal(g, _float_centroid(c))
+    assert np.allclose(g, c.astype(np.float64).mean(0), atol=1e-6)
+
+
+def test_funnel_rotation(tmp_path):
+    c = _cloud(401, seed=5)
+    f = tmp_path / "funnel.txt"
+    f.write_text("".join(f"{x!r} {y!r} {z!r}\n" for x, y, z in c.tolist()))
+    s = _scene_with_cloud("funnel", f)
+    pos, animated, anim = host.scene_boundary(s)
+    nw = host.scene_walls(s).shape[0]
+    got = pos[nw:].astype(np.float64)
+    g = _float_centroid(c).astype(np.float64)
+    d = c.astype(np.float64) - g
+    # +90 degrees about x through the centroid: (y, z) -> (-z, y), then
+    # + (1.25, 0.3, 1.5) (main.cu:1043-1070); float32 rounding only
+    want = np.stack([d[:, 0], -d[:, 2], d[:, 1]], 1) + g + [1.25, 0.3, 1.5]
+    assert np.abs(got - want).max() < 4e-7
+    # cos of the float-rounded right angle is not zero: its trace is visible
+    # where |y - g.y| is large against |z - g.z|
+    assert f32(np.cos(np.float64(f32(np.pi / 2)))) == f32(-4.371139e-08)
+    assert (animated[nw:] == 1).all() and anim.num_particles == len(c)
+    # the funnel keeps the default rotation centre (main.cu:1036-1041)
+    assert list(anim.rotation_center) == [0.0, 0.0, 0.0]
+    assert (anim.is_animating, anim.start_index) == (1, nw)
+
+
+REF_OBJ = Path("/root/reference/obj")
+
+
+@pytest.mark.skipif(not REF_OBJ.is_dir(), reason="reference point clouds not present")
+@pytest.mark.parametrize("name,count", [("excavator", 13045), ("funnel", 83581)])
+def test_reference_point_clouds(name, count):
+    fname = {"excavator": "bucket.txt", "funnel": "funnel.txt"}[name]
+    s = _scene_with_cloud(name, REF_OBJ / fname)
+    pos, animated, anim = host.scene_boundary(s)
+    nw = host.scene_walls(s).shape[0]
+    assert anim.num_particles == count and pos.shape[0] == nw + count
+    raw = np.loadtxt(REF_OBJ / fname, dtype=np.float32)
+    if name == "excavator":
+        assert np.array_equal(pos[nw:], raw + np.float32([2.5, 0.0, 0.5]))
+    else:
+        ext = pos[nw:].max(0) - pos[nw:].min(0)
+        rext = raw.max(0) - raw.min(0)
+        assert np.allclose(ext, [rext[0], rext[2], rext[1]], rtol=1e-5)
 
 
 def test_lattice_formula():
