@@ -39,7 +39,8 @@ class ggh_animation(C.Structure):
                 ("translation_direction", C.c_float * 3), ("translation_speed", C.c_float),
                 ("translation_complete", C.c_int), ("rotation_angle", C.c_float),
                 ("rotation_speed", C.c_float), ("rotation_center", C.c_float * 3),
-                ("translation_time", C.c_float), ("rotation_limit", C.c_float)]
+                ("translation_time", C.c_float), ("rotation_limit", C.c_float),
+                ("center_from_marked", C.c_int)]
 
 
 class HostError(RuntimeError):

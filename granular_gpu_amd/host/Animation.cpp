@@ -38,19 +38,33 @@ bool update_animation(GranularSystem &sys, AnimationState &a, float dt) {
     a.translation_complete = true;
     a.animation_time = 0.0f;
     const auto b = sys.get_boundaries();
-    const long long lo = a.start_index, hi = (long long)a.start_index + a.num_particles;
-    if (lo < 0 || hi > (long long)b->size())
+    long long lo = a.start_index, hi = (long long)a.start_index + a.num_particles;
+    if (a.center_from_marked) {
+      lo = 0;
+      hi = b->size();
+    } else if (lo < 0 || hi > (long long)b->size()) {
       throw std::out_of_range("update_animation: animated range outside the boundary set");
+    }
     if (hi > lo) {
       std::vector<float3> p((size_t)(hi - lo));
+      std::vector<int> m(a.center_from_marked ? p.size() : 0);
       if (hipDeviceSynchronize() != hipSuccess ||
           hipMemcpy(p.data(), b->get_pos_ptr() + lo, sizeof(float3) * p.size(),
-                    hipMemcpyDeviceToHost) != hipSuccess)
+                    hipMemcpyDeviceToHost) != hipSuccess ||
+          (!m.empty() && hipMemcpy(m.data(), b->get_is_animated_ptr(), sizeof(int) * m.size(),
+                                   hipMemcpyDeviceToHost) != hipSuccess))
         throw std::runtime_error("update_animation: device-to-host copy failed");
       float3 s = make_float3(0.0f, 0.0f, 0.0f);
-      for (const auto &q : p) s = make_float3(s.x + q.x, s.y + q.y, s.z + q.z);
-      const float n = (float)p.size();
-      a.rotation_center = make_float3(s.x / n, s.y / n, s.z / n);
+      int count = 0;
+      for (size_t i = 0; i < p.size(); ++i) {
+        if (!m.empty() && m[i] != 1) continue;
+        s = make_float3(s.x + p[i].x, s.y + p[i].y, s.z + p[i].z);
+        ++count;
+      }
+      if (count > 0) {
+        const float n = (float)count;
+        a.rotation_center = make_float3(s.x / n, s.y / n, s.z / n);
+      }
     }
   }
   if (a.translation_complete && a.rotation_angle < a.rotation_limit) {

@@ -105,6 +105,7 @@ void to_c(const AnimationState &a, ggh_animation *o) {
   o->rotation_center[2] = a.rotation_center.z;
   o->translation_time = a.translation_time;
   o->rotation_limit = a.rotation_limit;
+  o->center_from_marked = a.center_from_marked ? 1 : 0;
 }
 
 AnimationState from_c(const ggh_animation *o) {
@@ -123,6 +124,7 @@ AnimationState from_c(const ggh_animation *o) {
       make_float3(o->rotation_center[0], o->rotation_center[1], o->rotation_center[2]);
   a.translation_time = o->translation_time;
   a.rotation_limit = o->rotation_limit;
+  a.center_from_marked = o->center_from_marked != 0;
   return a;
 }
 

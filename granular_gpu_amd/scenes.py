@@ -184,3 +184,158 @@ def batch_box(n_side: int, seed: int = 1) -> Scene:
     max(0.024 n_side, 0.02 (n_side - 1) + 0.25), dt = 0.002, g = -9.8."""
     box = max(0.024 * n_side, 0.02 * (n_side - 1) + 0.25)
     return plain_box(n_side, float(np.float32(box)), seed=seed, dt=0.002, g=-9.8)
+
+
+def _largest_remainder(weights, total: int) -> np.ndarray:
+    """Integer shares of `total` proportional to `weights` (floors, then one
+    more for the largest fractional parts; ties go to the lower index)."""
+    w = np.asarray(weights, np.float64)
+    share = total * w / w.sum()
+    m = np.floor(share).astype(np.int64)
+    rest = total - int(m.sum())
+    order = np.lexsort((np.arange(w.size), -(share - m)))
+    m[order[:rest]] += 1
+    return m
+
+
+def cone_funnel(count: int, half_angle_deg: float = 30.0, outlet: float = 0.1,
+                height: float = 0.3, centre=(0.5, 0.3, 0.5)) -> np.ndarray:
+    """`count` boundary particles on the lateral surface of a conical funnel
+    with a vertical (y) axis: the outlet circle of diameter `outlet` centred
+    at `centre`, the wall opening upward at half_angle_deg from the axis to
+    `height` above the outlet.
+
+    Rings at equal slant steps (their number chosen so that the step along
+    the slant equals the spacing along a ring); ring k gets a share of
+    `count` proportional to its radius (largest remainder), and odd rings
+    are turned by half a spacing.  Formed in float64, rounded once."""
+    th = math.radians(half_angle_deg)
+    slant = height / math.cos(th)
+    r0 = 0.5 * outlet
+    r1 = r0 + slant * math.sin(th)
+    nr = max(2, int(round(math.sqrt(count * slant / (math.pi * (r0 + r1))))))
+    s = slant * np.arange(nr) / (nr - 1)
+    rk = r0 + s * math.sin(th)
+    m = _largest_remainder(rk, count)
+    cx, cy, cz = (float(v) for v in centre)
+    out = np.empty((count, 3), np.float64)
+    k0 = 0
+    for k in range(nr):
+        j = np.arange(m[k])
+        phi = 2.0 * math.pi * (j + 0.5 * (k % 2)) / max(int(m[k]), 1)
+        sl = slice(k0, k0 + m[k])
+        out[sl, 0] = cx + rk[k] * np.cos(phi)
+        out[sl, 1] = cy + s[k] * math.cos(th)
+        out[sl, 2] = cz + rk[k] * np.sin(phi)
+        k0 += m[k]
+    return out.astype(np.float32)
+
+
+def funnel_pour(n: int = 8000, cone_count: int = 12000, r: float = 0.01, box: float = 2.0,
+                seed: int = 2, dt: float = 0.002, g: float = -9.8,
+                half_angle_deg: float = 30.0, outlet: float = 0.1) -> Scene:
+    """Funnel pour: a cone_funnel of `cone_count` particles (axis at the
+    box centre in x and z, outlet at 0.3 box, height 0.3 box) inside BOX
+    walls, and a column of n particles at seeded uniform-random positions
+    in a cylinder above it: radius 0.8 of the cone's rim radius, from 0.02
+    above the rim, tall enough for (2.4 r)^3 of volume per particle."""
+    space = (box, box, box)
+    h = 0.3 * box
+    centre = (0.5 * box, 0.3 * box, 0.5 * box)
+    cone = cone_funnel(cone_count, half_angle_deg, outlet, h, centre)
+    rim = 0.5 * outlet + h * math.tan(math.radians(half_angle_deg))
+    R = 0.8 * rim
+    y0 = centre[1] + h + 0.02
+    tall = n * (2.4 * r) ** 3 / (math.pi * R * R)
+    if y0 + tall > 0.99 * box:
+        raise ValueError("funnel_pour: the column does not fit in the box")
+    u = uniform01(seed, 3 * n).reshape(n, 3)
+    rad = R * np.sqrt(u[:, 0])
+    phi = 2.0 * math.pi * u[:, 1]
+    pos = np.stack([centre[0] + rad * np.cos(phi), y0 + tall * u[:, 2],
+                    centre[2] + rad * np.sin(phi)], 1).astype(np.float32)
+    walls = box_walls(space, CELL_LENGTH, "box")
+    return Scene(
+        name=f"funnel_pour_{n}_cone_{cone_count}", pos=pos, vel=np.zeros((n, 3), np.float32),
+        mass=np.full(n, radius_mass(r), np.float32),
+        boundary=np.concatenate([walls, cone]).astype(np.float32), space=space,
+        cell_length=CELL_LENGTH, cells=cell_counts(space), dt=np.float32(dt),
+        gravity=(0.0, float(np.float32(g)), 0.0), density=DENSITY_R001, radius=r)
+
+
+def bucket_scoop(count: int, depth: float = 0.2, width: float = 0.3, height: float = 0.15,
+                 corner=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """About `count` boundary particles on an open scoop: a bottom plate
+    (x in [0, depth], z in [0, width]), a back plate at x = depth and two
+    side plates at z = 0 and z = width (y in [0, height]); the open mouth
+    faces -x.  One square grid of spacing sqrt(area / count); each shared
+    edge belongs to one plate only.  `corner` is the (x, y, z) of the
+    scoop's origin."""
+    area = depth * width + height * width + 2.0 * depth * height
+    s = math.sqrt(area / count)
+    nx = max(2, int(round(depth / s)) + 1)
+    ny = max(2, int(round(height / s)) + 1)
+    nz = max(2, int(round(width / s)) + 1)
+    xs = np.linspace(0.0, depth, nx)
+    ys = np.linspace(0.0, height, ny)
+    zs = np.linspace(0.0, width, nz)
+    pts = []
+    X, Z = np.meshgrid(xs, zs, indexing="ij")                       # bottom
+    pts.append(np.stack([X.ravel(), np.zeros(X.size), Z.ravel()], 1))
+    Y, Z = np.meshgrid(ys[1:], zs, indexing="ij")                   # back
+    pts.append(np.stack([np.full(Y.size, depth), Y.ravel(), Z.ravel()], 1))
+    X, Y = np.meshgrid(xs[:-1], ys[1:], indexing="ij")              # sides
+    for z in (0.0, width):
+        pts.append(np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], 1))
+    out = np.concatenate(pts) + np.asarray(corner, np.float64)
+    return out.astype(np.float32)
+
+
+@dataclass
+class DigScene:
+    scene: Scene
+    animated: np.ndarray   # (nb,) int32: 1 for the bucket
+    script: dict           # animation-state fields of the scripted arc
+
+
+def excavator_dig(bed=(0.6, 0.3), fill: float = 0.12, r: float = 0.01,
+                  bucket_count: int = 3000, seed: int = 5, dt: float = 0.002,
+                  g: float = -9.8) -> DigScene:
+    """Excavator dig: a bed of particles on a 2r lattice (jitter 0.1 r,
+    seeded) covering a bed[0] x bed[1] floor up to `fill`, in BOX walls of
+    (bed[0], 4 fill, bed[1]), and a bucket_scoop (depth 0.3 bed[0], width
+    0.6 bed[1], height fill) hanging 0.02 above the bed near the +x wall,
+    mouth towards -x.  The scripted arc (animation fields in `script`):
+    0.25 bed[0] (0.5 m for a 2 m bed) in 1 s, 15 degrees below -x into the
+    bed, then 90 degrees about z at 45 degrees/s about the bucket's centroid,
+    turning the mouth up.  The cell length and the wall spacing scale with
+    r (4.04 r); masses follow radius_mass(r)."""
+    bx, bz = bed
+    space = (bx, 4.0 * fill, bz)
+    spacing = 2.0 * r
+    lo = 0.005 * np.asarray(space) + 0.01 + r
+    nxl = int((bx - 2 * lo[0]) / spacing) + 1
+    nzl = int((bz - 2 * lo[2]) / spacing) + 1
+    nyl = max(1, int((fill - lo[1]) / spacing) + 1)
+    pos = jittered_lattice(nxl, nyl, nzl, spacing, (lo[0], lo[1], lo[2]), 0.1 * r, seed)
+    n = pos.shape[0]
+    d, w, h = 0.3 * bx, 0.6 * bz, fill
+    top = float(pos[:, 1].max()) + r
+    bucket = bucket_scoop(bucket_count, d, w, h, (bx - d - 0.05 * bx, top + 0.02, 0.5 * (bz - w)))
+    L = np.float32(float(CELL_LENGTH) * r / 0.01)
+    walls = box_walls(space, L, "box")
+    boundary = np.concatenate([walls, bucket]).astype(np.float32)
+    animated = np.zeros(boundary.shape[0], np.int32)
+    animated[walls.shape[0]:] = 1
+    a = math.radians(15.0)
+    script = dict(is_animating=1, start_index=walls.shape[0], num_particles=bucket.shape[0],
+                  translation_direction=(-math.cos(a), -math.sin(a), 0.0),
+                  translation_speed=0.25 * bx,
+                  translation_time=1.0, rotation_speed=45.0, rotation_limit=90.0,
+                  center_from_marked=1)
+    sc = Scene(name=f"excavator_dig_{n}_bucket_{bucket.shape[0]}", pos=pos,
+               vel=np.zeros((n, 3), np.float32), mass=np.full(n, radius_mass(r), np.float32),
+               boundary=boundary, space=space, cell_length=L,
+               cells=cell_counts(space, L), dt=np.float32(dt),
+               gravity=(0.0, float(np.float32(g)), 0.0), density=DENSITY_R001, radius=r)
+    return DigScene(sc, animated, script)

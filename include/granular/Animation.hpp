@@ -21,7 +21,8 @@ void animate_boundary(GranularSystem &sys, float3 translation, float angle_deg, 
 //    complete: move by translation_direction * translation_speed * dt;
 //  - on the first call past translation_time: mark the translation complete,
 //    reset animation_time, and set rotation_center to the float centroid of
-//    boundary particles [start_index, start_index + num_particles);
+//    boundary particles [start_index, start_index + num_particles) (or of
+//    the marked particles, with center_from_marked), in set order;
 //  - then, while rotation_angle < rotation_limit: rotation_angle +=
 //    rotation_speed * dt and rotate by -rotation_speed * dt degrees; the
 //    call that reaches the limit ends the animation and moves nothing.

@@ -56,7 +56,11 @@ std::vector<float3> boundary_walls(const SceneConfig &c);
 // defaults are the reference's AnimationState (main.cu:100-111);
 // translation_time and rotation_limit are the reference's hard-coded 2.5 s
 // and 45 degrees of updateAnimation (main.cu:1219, 1255-1262), made
-// settable for scripted moves.
+// settable for scripted moves.  The rotation centre is recomputed, when the
+// translation ends, over the index range [start_index, start_index +
+// num_particles) as the reference does; the boundary is cell-sorted by then,
+// so that range no longer holds the cloud.  center_from_marked = true takes
+// the centroid of the particles marked animated instead.
 struct AnimationState {
   bool is_animating = false;
   float animation_time = 0.0f;
@@ -70,6 +74,7 @@ struct AnimationState {
   float3 rotation_center = make_float3(0.0f, 0.0f, 0.0f);
   float translation_time = 2.5f;
   float rotation_limit = 45.0f;
+  bool center_from_marked = false;
 };
 
 // The boundary set of a scene: walls, then (EXCAVATOR, FUNNEL) the point

@@ -64,6 +64,7 @@ typedef struct ggh_animation {
   float rotation_center[3];
   float translation_time;
   float rotation_limit;
+  int center_from_marked; /* 1: centre of the marked particles, not the range */
 } ggh_animation;
 void ggh_animation_default(ggh_animation *a);
 

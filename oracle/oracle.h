@@ -179,6 +179,7 @@ typedef struct oracle_animation {
   float rotation_center[3];
   float translation_time;
   float rotation_limit;
+  int center_from_marked;
 } oracle_animation;
 void oracle_rotation_cs(float angle_deg, float *c, float *s);
 /* animateParticles on n AoS positions: marked (== 1) particles translate,

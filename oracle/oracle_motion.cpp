@@ -53,7 +53,10 @@ int oracle_update_animation(oracle_animation *a, int n, float *pos, const int *a
     a->animation_time = 0.0f;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     int count = 0;
-    for (int i = a->start_index; i < a->start_index + a->num_particles; ++i) {
+    const int lo = a->center_from_marked ? 0 : a->start_index;
+    const int hi = a->center_from_marked ? n : a->start_index + a->num_particles;
+    for (int i = lo; i < hi; ++i) {
+      if (a->center_from_marked && animated[i] != 1) continue;
       sx += pos[3 * i];
       sy += pos[3 * i + 1];
       sz += pos[3 * i + 2];
