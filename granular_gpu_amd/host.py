@@ -71,6 +71,9 @@ def load() -> C.CDLL:
         "ggh_scene_boundary": (C.c_int, [S, vp, vp, C.c_int, C.POINTER(ggh_animation)]),
         "ggh_system_animate": (C.c_int, [vp, fp, C.c_float, fp]),
         "ggh_system_update_animation": (C.c_int, [vp, C.POINTER(ggh_animation), C.c_float]),
+        "ggh_system_explode": (C.c_int, [vp, C.c_float, fp]),
+        "ggh_system_pick": (C.c_int, [vp, fp, C.c_float, ip, C.c_int]),
+        "ggh_system_move_picked": (C.c_int, [vp, ip, C.c_int, fp, fp, fp]),
         "ggh_system_create": (vp, [vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, fp, C.c_float,
                                    C.c_float, fp, ip, C.c_float, C.c_float, C.c_int, C.c_int,
                                    C.c_int]),
@@ -291,6 +294,33 @@ class HostSystem:
         c = (C.c_float * 3)(*[float(v) for v in center])
         _chk(self.lib, self.lib.ggh_system_animate(self.h, t, float(angle_deg), c),
              "ggh_system_animate")
+
+    def explode(self, force: float = 50.0) -> np.ndarray:
+        """Solver::trigger_explosion on the granular set; returns the centre
+        of mass it used."""
+        com = (C.c_float * 3)()
+        _chk(self.lib, self.lib.ggh_system_explode(self.h, float(force), com),
+             "ggh_system_explode")
+        return np.array(com, np.float32)
+
+    def pick(self, center, radius: float) -> np.ndarray:
+        """Indices of the granular particles within radius of center,
+        ascending."""
+        c = (C.c_float * 3)(*[float(v) for v in center])
+        n = _chk(self.lib, self.lib.ggh_system_pick(self.h, c, float(radius), None, 0),
+                 "ggh_system_pick")
+        out = np.zeros(max(n, 1), np.int32)
+        _chk(self.lib, self.lib.ggh_system_pick(self.h, c, float(radius),
+                                                out.ctypes.data_as(C.POINTER(C.c_int)), n),
+             "ggh_system_pick")
+        return out[:n]
+
+    def move_picked(self, indices, delta, lo=(0.01, 0.01, 0.01), hi=(0.99, 0.99, 0.99)) -> int:
+        idx = np.ascontiguousarray(indices, np.int32)
+        f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+        return _chk(self.lib, self.lib.ggh_system_move_picked(
+            self.h, idx.ctypes.data_as(C.POINTER(C.c_int)), idx.shape[0], f3(delta), f3(lo),
+            f3(hi)), "ggh_system_move_picked")
 
     def update_animation(self, anim: ggh_animation, dt: float) -> bool:
         """One updateAnimation(dt) call advancing `anim`; True if the

@@ -300,6 +300,20 @@ void Solver::upsampled_update(std::shared_ptr<GranularParticles> &particles,
         "gg_upsample_plain");
 }
 
+float3 Solver::trigger_explosion(std::shared_ptr<GranularParticles> &particles,
+                                 float explosion_force) {
+  float3 com = make_float3(0.0f, 0.0f, 0.0f);
+  const int n = (int)particles->size();
+  if (n == 0) return com;
+  DArray<unsigned int> scratch((unsigned int)((gg_explosion_scratch_bytes(n) + 3) / 4));
+  gg_particles s = particles->view();
+  gg_float3 c{0.0f, 0.0f, 0.0f};
+  check(gg_explosion(&s, explosion_force, scratch.addr(), 4 * (size_t)scratch.length(), &c,
+                     nullptr),
+        "gg_explosion");
+  return make_float3(c.x, c.y, c.z);
+}
+
 void Solver::save_particle_stats(const std::string &filename) {
   std::ofstream outfile(filename);
   if (!outfile.is_open()) {

@@ -347,6 +347,78 @@ int ggh_system_animate(ggh_system *h, const float translation[3], float angle_de
   }
 }
 
+int ggh_system_explode(ggh_system *h, float force, float com_out[3]) {
+  try {
+    if (!h) throw std::invalid_argument("null handle");
+    auto p = h->sys->get_particles_non_const();
+    const float3 c = h->sys->get_solver().trigger_explosion(p, force);
+    if (com_out) {
+      com_out[0] = c.x;
+      com_out[1] = c.y;
+      com_out[2] = c.z;
+    }
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_pick(const ggh_system *h, const float center[3], float radius, int *indices,
+                    int max) {
+  try {
+    if (!h || !center) throw std::invalid_argument("null argument");
+    auto p = h->sys->get_particles_non_const();
+    const int n = (int)p->size();
+    if (n == 0) return 0;
+    DArray<int> idx((unsigned int)n), count(1u);
+    DArray<unsigned int> scratch((unsigned int)((gg_pick_scratch_bytes(n) + 3) / 4));
+    const gg_particles s = p->view();
+    const int rc = gg_pick_sphere(&s, gg_float3{center[0], center[1], center[2]}, radius,
+                                  idx.addr(), count.addr(), scratch.addr(),
+                                  4 * (size_t)scratch.length(), nullptr);
+    if (rc != GG_OK) throw std::runtime_error(std::string("gg_pick_sphere: ") + gg_status_string(rc));
+    int c = 0;
+    if (hipMemcpy(&c, count.addr(), sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("copy failed");
+    if (indices && max > 0 && c > 0 &&
+        hipMemcpy(indices, idx.addr(), sizeof(int) * (size_t)std::min(c, max),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("copy failed");
+    return c;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_move_picked(ggh_system *h, const int *indices, int count, const float delta[3],
+                           const float lo[3], const float hi[3]) {
+  try {
+    if (!h || !delta || !lo || !hi || count < 0 || (count > 0 && !indices))
+      throw std::invalid_argument("bad arguments");
+    auto p = h->sys->get_particles_non_const();
+    const int n = (int)p->size();
+    if (count > n) throw std::invalid_argument("more picked indices than particles");
+    for (int k = 0; k < count; ++k)
+      if (indices[k] < 0 || indices[k] >= n) throw std::out_of_range("picked index out of range");
+    if (count == 0) return 0;
+    DArray<int> idx((unsigned int)count), dc(1u);
+    if (hipMemcpy(idx.addr(), indices, sizeof(int) * (size_t)count, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(dc.addr(), &count, sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      throw std::runtime_error("copy failed");
+    gg_particles s = p->view();
+    const int rc = gg_move_picked(&s, idx.addr(), dc.addr(), count,
+                                  gg_float3{delta[0], delta[1], delta[2]},
+                                  gg_float3{lo[0], lo[1], lo[2]}, gg_float3{hi[0], hi[1], hi[2]},
+                                  nullptr);
+    if (rc != GG_OK || hipDeviceSynchronize() != hipSuccess)
+      throw std::runtime_error(std::string("gg_move_picked: ") + gg_status_string(rc));
+    return count;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
 int ggh_system_update_animation(ggh_system *h, ggh_animation *a, float dt) {
   try {
     if (!h || !a) throw std::invalid_argument("null argument");

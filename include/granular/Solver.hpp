@@ -104,6 +104,11 @@ class Solver {
     _buffer_remove.resize((unsigned int)size);
   }
 
+  // Solver::trigger_explosion (Solver.cu:1411-1442): radial impulse
+  // force / (1 + d^2) / m away from the centre of mass (gg_explosion; the
+  // centre is summed in double on the device).  Returns the centre.
+  float3 trigger_explosion(std::shared_ptr<GranularParticles> &particles, float explosion_force);
+
   void save_particle_stats(const std::string &filename);
   const std::vector<std::tuple<int, int, int>> &get_particle_stats() const {
     return _particle_stats;

@@ -445,6 +445,36 @@ int gg_step_graph_launch(gg_step_graph *g, hipStream_t stream);
 void gg_step_graph_destroy(gg_step_graph *g);
 
 /* ---- device-array primitives (host DArray class) --------------------------- */
+/* ---- interactive extras ------------------------------------------------- */
+/* Explosion impulse.  Centre of mass c = sum(m p) / sum(m) accumulated in
+ * double (fixed-order per-block sums, then the blocks in order; 0 if the
+ * total mass is not positive), rounded to float; then for every particle
+ * with d = p - c, |d| >= 1e-4:  v += (d / |d|) * (force / (1 + |d|^2)) / m.
+ * scratch: gg_explosion_scratch_bytes(n) bytes of device memory.  com_out
+ * (host, optional) receives c; asking for it synchronizes the stream.
+ * Replaces: Solver::trigger_explosion + apply_explosion_force
+ * (Solver.cu:1389-1442), which sums the centre of mass sequentially in
+ * float on the host. */
+size_t gg_explosion_scratch_bytes(int n);
+int gg_explosion(gg_particles *set, float force, void *scratch, size_t scratch_bytes,
+                 gg_float3 *com_out, hipStream_t stream);
+
+/* Sphere picking: the indices i with sqrtf(|p_i - center|^2) <= radius,
+ * ascending, to indices (device, room for n) and their number to *count
+ * (device).  scratch: gg_pick_scratch_bytes(n) bytes.
+ * Replaces: markParticlesInSphere (main.cu:1375-1400), whose atomic append
+ * gives the indices in no fixed order. */
+size_t gg_pick_scratch_bytes(int n);
+int gg_pick_sphere(const gg_particles *set, gg_float3 center, float radius, int *indices,
+                   int *count, void *scratch, size_t scratch_bytes, hipStream_t stream);
+/* Moves the first min(*count, max_count) picked particles (indices in
+ * [0, n), device) by delta and clamps each coordinate to [lo, hi] with
+ * fmaxf(lo, fminf(p + delta, hi)).  max_count <= n sizes the launch.
+ * Replaces: updatePickedParticlesPositions (main.cu:1402-1426), whose bounds
+ * are fixed at 0.01 and 0.99. */
+int gg_move_picked(gg_particles *set, const int *indices, const int *count, int max_count,
+                   gg_float3 delta, gg_float3 lo, gg_float3 hi, hipStream_t stream);
+
 /* Stable compaction (DArray<T>::compact, DArray.hpp:62-122): copies the
  * elements i of src with flags[i] != 1, in order, to dst (no aliasing);
  * elements are elem_words 32-bit words; the kept count is written to the

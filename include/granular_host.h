@@ -108,6 +108,18 @@ int ggh_system_animate(ggh_system *h, const float translation[3], float angle_de
 /* One updateAnimation(dt) call (main.cu:1210-1277) advancing *a; returns 1
  * if the boundary moved, 0 if not, negative on error. */
 int ggh_system_update_animation(ggh_system *h, ggh_animation *a, float dt);
+/* Interactive extras on the granular set.  Explosion: Solver::
+ * trigger_explosion (Solver.cu:1411), the 'e' key of main.cu:1905 uses
+ * force 50; com_out (optional) receives the centre of mass.  Picking
+ * (main.cu:1375-1426): ggh_system_pick writes up to max indices within
+ * radius of center, ascending, and returns their number;
+ * ggh_system_move_picked moves those particles by delta, clamped to
+ * [lo, hi] per coordinate (the reference clamps to [0.01, 0.99]). */
+int ggh_system_explode(ggh_system *h, float force, float com_out[3]);
+int ggh_system_pick(const ggh_system *h, const float center[3], float radius, int *indices,
+                    int max);
+int ggh_system_move_picked(ggh_system *h, const int *indices, int count, const float delta[3],
+                           const float lo[3], const float hi[3]);
 int ggh_system_frame_times(const ggh_system *h, float *out, int max);
 /* Rows (merged, split, N before) of the adaptive passes; returns the count. */
 int ggh_system_particle_stats(const ggh_system *h, int *out, int max_rows);

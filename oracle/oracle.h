@@ -190,6 +190,14 @@ void oracle_animate(int n, float *pos, const int *animated, float tx, float ty, 
 int oracle_update_animation(oracle_animation *a, int n, float *pos, const int *animated,
                             float dt);
 
+/* ---- interactive extras (oracle_motion.cpp) ------------------------------ */
+void oracle_center_of_mass(int n, const float *pos, const float *mass, float *com);
+void oracle_explode(int n, const float *pos, float *vel, const float *mass, const float *com,
+                    float force);
+int oracle_pick_sphere(int n, const float *pos, const float *center, float radius, int *out);
+void oracle_move_picked(float *pos, const int *idx, int count, const float *delta,
+                        const float *lo, const float *hi);
+
 /* Number of OpenMP threads the oracle uses (0 = default).                */
 void oracle_set_threads(int t);
 int oracle_get_threads(void);

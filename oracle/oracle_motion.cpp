@@ -1,6 +1,7 @@
 // CPU oracle -- TEST INFRASTRUCTURE ONLY.  Kinematic boundary motion:
 // restates animateParticles (main.cu:1175-1208) and updateAnimation
-// (main.cu:1210-1277) of the reference on host arrays.
+// (main.cu:1210-1277) of the reference on host arrays; and the interactive
+// extras (explosion impulse, sphere picking and the clamped move).
 #include <cmath>
 
 #include "oracle.h"
@@ -78,6 +79,69 @@ int oracle_update_animation(oracle_animation *a, int n, float *pos, const int *a
   oracle_animate(n, pos, animated, t[0], t[1], t[2], rotation, a->rotation_center[0],
                  a->rotation_center[1]);
   return 1;
+}
+
+}  // extern "C"
+
+// ---- interactive extras (Solver.cu:1389-1442, main.cu:1375-1426) ----------
+
+extern "C" {
+
+// Centre of mass sum(m p) / sum(m), accumulated in double in index order and
+// rounded to float; 0 when the total mass is not positive.
+void oracle_center_of_mass(int n, const float *pos, const float *mass, float *com) {
+  double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
+  for (int i = 0; i < n; ++i) {
+    x += (double)pos[3 * i] * mass[i];
+    y += (double)pos[3 * i + 1] * mass[i];
+    z += (double)pos[3 * i + 2] * mass[i];
+    m += mass[i];
+  }
+  if (m > 0.0) {
+    com[0] = (float)(x / m);
+    com[1] = (float)(y / m);
+    com[2] = (float)(z / m);
+  } else {
+    com[0] = com[1] = com[2] = 0.0f;
+  }
+}
+
+// apply_explosion_force (Solver.cu:1389-1409) about a given centre.
+void oracle_explode(int n, const float *pos, float *vel, const float *mass, const float *com,
+                    float force) {
+  for (int i = 0; i < n; ++i) {
+    const float dx = pos[3 * i] - com[0], dy = pos[3 * i + 1] - com[1],
+                dz = pos[3 * i + 2] - com[2];
+    const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (dist < 1e-4f) continue;
+    const float ux = dx / dist, uy = dy / dist, uz = dz / dist;
+    const float mag = force / (1.0f + dist * dist);
+    vel[3 * i] += (ux * mag) / mass[i];
+    vel[3 * i + 1] += (uy * mag) / mass[i];
+    vel[3 * i + 2] += (uz * mag) / mass[i];
+  }
+}
+
+// markParticlesInSphere (main.cu:1375-1400) with the indices in ascending
+// order; returns their number.
+int oracle_pick_sphere(int n, const float *pos, const float *center, float radius, int *out) {
+  int c = 0;
+  for (int i = 0; i < n; ++i) {
+    const float dx = pos[3 * i] - center[0], dy = pos[3 * i + 1] - center[1],
+                dz = pos[3 * i + 2] - center[2];
+    if (sqrtf(dx * dx + dy * dy + dz * dz) <= radius) out[c++] = i;
+  }
+  return c;
+}
+
+// updatePickedParticlesPositions (main.cu:1402-1426) with bounds lo, hi.
+void oracle_move_picked(float *pos, const int *idx, int count, const float *delta,
+                        const float *lo, const float *hi) {
+  for (int k = 0; k < count; ++k)
+    for (int a = 0; a < 3; ++a) {
+      float *p = pos + 3 * (long long)idx[k] + a;
+      *p = fmaxf(lo[a], fminf(*p + delta[a], hi[a]));
+    }
 }
 
 }  // extern "C"

@@ -103,6 +103,13 @@ def load() -> C.CDLL:
     lib.oracle_animate.restype = None
     lib.oracle_update_animation.argtypes = [C.POINTER(oracle_animation), C.c_int, fp, ip,
                                             C.c_float]
+    lib.oracle_center_of_mass.argtypes = [C.c_int, fp, fp, fp]
+    lib.oracle_center_of_mass.restype = None
+    lib.oracle_explode.argtypes = [C.c_int, fp, fp, fp, fp, C.c_float]
+    lib.oracle_explode.restype = None
+    lib.oracle_pick_sphere.argtypes = [C.c_int, fp, fp, C.c_float, ip]
+    lib.oracle_move_picked.argtypes = [fp, ip, C.c_int, fp, fp, fp]
+    lib.oracle_move_picked.restype = None
     lib.oracle_set_threads.argtypes = [C.c_int]
     lib.oracle_get_threads.restype = C.c_int
     _lib = lib
@@ -423,3 +430,37 @@ def update_animation(state: oracle_animation, pos, animated, dt):
     return bool(load().oracle_update_animation(
         C.byref(state), pos.shape[0], pos.reshape(-1),
         np.ascontiguousarray(animated, np.int32), float(dt)))
+
+
+def center_of_mass(pos, mass):
+    pos = np.ascontiguousarray(pos, np.float32)
+    com = np.zeros(3, np.float32)
+    load().oracle_center_of_mass(pos.shape[0], pos.reshape(-1),
+                                 np.ascontiguousarray(mass, np.float32), com)
+    return com
+
+
+def explode(pos, vel, mass, com, force):
+    """Velocities after the explosion impulse about `com`."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32).copy()
+    load().oracle_explode(pos.shape[0], pos.reshape(-1), vel.reshape(-1),
+                          np.ascontiguousarray(mass, np.float32),
+                          np.ascontiguousarray(com, np.float32), float(force))
+    return vel
+
+
+def pick_sphere(pos, center, radius):
+    pos = np.ascontiguousarray(pos, np.float32)
+    out = np.zeros(max(pos.shape[0], 1), np.int32)
+    c = load().oracle_pick_sphere(pos.shape[0], pos.reshape(-1),
+                                  np.ascontiguousarray(center, np.float32), float(radius), out)
+    return out[:c].copy()
+
+
+def move_picked(pos, idx, delta, lo=(0.01, 0.01, 0.01), hi=(0.99, 0.99, 0.99)):
+    pos = np.ascontiguousarray(pos, np.float32).copy()
+    idx = np.ascontiguousarray(idx, np.int32)
+    f3 = lambda v: np.ascontiguousarray(v, np.float32)
+    load().oracle_move_picked(pos.reshape(-1), idx, idx.shape[0], f3(delta), f3(lo), f3(hi))
+    return pos

@@ -184,6 +184,34 @@ def test_boundary_animation_matches_oracle(gpu, tmp_path):
     sysm.close()
 
 
+def test_interactive_extras_match_oracle(gpu):
+    # explosion impulse, sphere picking and the clamped move, against the
+    # oracle on the system's own arrays
+    from granular_gpu_amd import host
+    _log("interactive extras: gg_explosion, gg_pick_sphere, gg_move_picked")
+    s = scenes.plain_box(12, 1.0, dt=0.002, g=-9.8)
+    sysm = host.HostSystem.from_scene(s)
+    sysm.step(3)
+    pos, vel = sysm.read("granular", "pos"), sysm.read("granular", "vel")
+    mass = sysm.read("granular", "mass")
+    com = sysm.explode(50.0)
+    ocom = po.center_of_mass(pos, mass)
+    assert (np.abs(com - ocom) <= np.spacing(np.abs(ocom))).all()
+    assert np.array_equal(sysm.read("granular", "vel"), po.explode(pos, vel, mass, com, 50.0))
+    c, r = pos[100] + np.float32([0.01, 0.0, -0.01]), 0.05
+    idx = sysm.pick(c, r)
+    assert idx.size > 10 and np.array_equal(idx, po.pick_sphere(pos, c, r))
+    assert sysm.move_picked(idx, (0.2, 0.9, -0.05)) == idx.size
+    assert np.array_equal(sysm.read("granular", "pos"),
+                          po.move_picked(pos, idx, (0.2, 0.9, -0.05)))
+    with pytest.raises(host.HostError, match="out of range"):
+        sysm.move_picked([0, s.n], (0.0, 0.0, 0.0))
+    assert sysm.pick(c, -1.0).size == 0
+    sysm.step(2)
+    assert np.isfinite(sysm.read("granular", "pos")).all()
+    sysm.close()
+
+
 def test_system_outputs(gpu, tmp_path):
     # VTK files and the checkpoint carry the system's own arrays
     from granular_gpu_amd import host

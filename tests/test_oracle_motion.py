@@ -109,6 +109,64 @@ def test_update_animation_timeline():
     assert np.array_equal(pos, frozen)
 
 
+def test_explosion_closed_forms():
+    # two equal masses at +-a on x: centre 0, each pushed out by
+    # F / (1 + a^2) / m; a particle on the centre is left alone
+    a, F, m = 0.3, 50.0, 2.0
+    pos = np.float32([[-a, 0.0, 0.0], [a, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    mass = np.float32([m, m, 1.0])
+    com = po.center_of_mass(pos, mass)
+    assert np.array_equal(com, np.zeros(3, np.float32))
+    vel = po.explode(pos, np.zeros_like(pos), mass, com, F)
+    dv = F / (1 + a * a) / m
+    assert np.allclose(vel[0], [-dv, 0, 0], rtol=1e-6) and np.allclose(vel[1], [dv, 0, 0], rtol=1e-6)
+    assert np.array_equal(vel[2], [0, 0, 0])
+
+
+def test_explosion_radial_and_momentum():
+    rng = np.random.default_rng(7)
+    pos = rng.random((500, 3)).astype(np.float32)
+    mass = (1.0 + 3.0 * rng.random(500)).astype(np.float32)
+    com = po.center_of_mass(pos, mass)
+    want = (pos.astype(np.float64) * mass[:, None]).sum(0) / mass.astype(np.float64).sum()
+    assert np.abs(com - want).max() < 1e-7
+    v0 = rng.standard_normal((500, 3)).astype(np.float32)
+    vel = po.explode(pos, v0, mass, com, 50.0)
+    dv = (vel - v0).astype(np.float64)
+    d = pos.astype(np.float64) - com
+    r = np.linalg.norm(d, axis=1)
+    # along the radius, magnitude F / (1 + r^2) / m
+    assert np.abs(np.cross(dv, d)).max() / np.abs(dv).max() < 1e-5
+    assert np.allclose(np.linalg.norm(dv, axis=1), 50.0 / (1 + r * r) / mass, rtol=1e-5)
+    # mirror-symmetric pairs about the centre: total momentum change is zero
+    sym = np.concatenate([d[:50], -d[:50]]) + com
+    ms = np.concatenate([mass[:50], mass[:50]])
+    dvs = po.explode(sym.astype(np.float32), np.zeros((100, 3), np.float32), ms, com, 50.0)
+    p = (dvs.astype(np.float64) * ms[:, None]).sum(0)
+    assert np.abs(p).max() < 1e-3 * np.abs(dvs * ms[:, None]).sum(0).max()
+
+
+def test_pick_sphere_brute_force():
+    rng = np.random.default_rng(3)
+    pos = rng.random((3000, 3)).astype(np.float32)
+    c, r = np.float32([0.4, 0.5, 0.6]), 0.2
+    got = po.pick_sphere(pos, c, r)
+    d = np.linalg.norm(pos.astype(np.float64) - c.astype(np.float64), axis=1)
+    sure_in = np.nonzero(d < r - 1e-6)[0]
+    sure_out = np.nonzero(d > r + 1e-6)[0]
+    assert np.all(np.diff(got) > 0)
+    assert set(sure_in) <= set(got) and not (set(sure_out) & set(got))
+    assert po.pick_sphere(pos, c, 0.0).size == 0
+
+
+def test_move_picked_clamps():
+    pos = np.float32([[0.5, 0.5, 0.5], [0.98, 0.02, 0.5], [0.3, 0.3, 0.3]])
+    got = po.move_picked(pos, [0, 1], (0.05, -0.05, 0.0))
+    assert np.allclose(got[0], [0.55, 0.45, 0.5])
+    assert np.array_equal(got[1], np.float32([0.99, 0.01, 0.5]))
+    assert np.array_equal(got[2], pos[2])
+
+
 def test_update_animation_scripted_arc():
     # a scripted move: 0.5 m along x in 1 s, then 90 degrees at 45 deg/s
     pos, _ = _points(64, seed=4)
