@@ -131,16 +131,60 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---- features ---------------------------------------------------------------
-// Stencil cells nearest-first (centre, 6 faces, 12 edges, 8 corners) as
-// stencil_cell indices m = 9 (dx+1) + 3 (dy+1) + (dz+1).  The 16 smallest
-// keys are a set, so the visiting order does not change them; visiting near
-// cells first means most later candidates are rejected by one comparison.
-__constant__ int kNearFirst[27] = {13, 4,  10, 12, 14, 16, 22, 1,  3,  5,  7,  9,  11, 15,
-                                   17, 19, 21, 23, 25, 0,  2,  6,  8,  18, 20, 24, 26};
+// Stencil rows (the three cells dz = -1..1 of one (dx, dy), one contiguous
+// index range) nearest-first: centre, the four faces, the four corners, as
+// r = 3 (dx + 1) + (dy + 1).  The 16 smallest keys are a set, so the
+// visiting order does not change them; near rows first let later chunks be
+// skipped when no lane can improve.
+__constant__ int kRowNearFirst[9] = {4, 1, 3, 5, 7, 0, 2, 6, 8};
+
+// Compare-exchange of 64-bit keys: a <- min, b <- max.
+__device__ __forceinline__ void kv_ce(unsigned long long &a, unsigned long long &b) {
+  const bool sw = b < a;
+  const unsigned long long lo = sw ? b : a, hi = sw ? a : b;
+  a = lo;
+  b = hi;
+}
+
+// Bitonic sort of 16 keys, ascending: 80 compare-exchanges, no branches.
+__device__ __forceinline__ void kv_sort16(unsigned long long (&v)[16]) {
+#pragma unroll
+  for (int k = 2; k <= 16; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int l = i ^ j;
+        if (l > i) {
+          if ((i & k) == 0)
+            kv_ce(v[i], v[l]);
+          else
+            kv_ce(v[l], v[i]);
+        }
+      }
+}
+
+// top <- the 16 smallest keys of top and c (both ascending), ascending:
+// min(top[i], c[15 - i]) holds them as a bitonic sequence, which a
+// bitonic merger sorts (16 minima + 32 compare-exchanges).
+__device__ __forceinline__ void kv_merge16(unsigned long long (&top)[16],
+                                           const unsigned long long (&c)[16]) {
+#pragma unroll
+  for (int i = 0; i < 16; ++i) top[i] = c[15 - i] < top[i] ? c[15 - i] : top[i];
+#pragma unroll
+  for (int j = 8; j > 0; j >>= 1)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int l = i ^ j;
+      if (l > i) kv_ce(top[i], top[l]);
+    }
+}
 
 // One thread per coarse particle: the 16 smallest (d2, j) of the stencil as
-// 64-bit keys (d2 bits << 32 | j; d2 >= 0 orders like its bits), kept sorted
-// in registers by a branchless insertion.
+// 64-bit keys (d2 bits << 32 | j; d2 >= 0 orders like its bits).  The
+// candidates come in chunks of 16; a chunk is sorted by a sorting network
+// and merged into the kept 16, the same work on every lane (a per-candidate
+// insertion runs whenever any lane of the wave inserts).
 // GROUPED = 0: feat[i][96] (sample-major, the training layout);
 // GROUPED = 1: feat[g][i][4] for the 24 groups g of 4 consecutive features
 // (the inference layout: a wave's stores and the network's B-operand loads
@@ -156,30 +200,61 @@ __global__ void __launch_bounds__(256)
   const Packed3 xi = ld3(pos, i);
   const float xx = xi.x, xy = xi.y, xz = xi.z;
   constexpr unsigned long long kNone = ~0ull;
+  static_assert(kKnn == 16, "the selection network is built for 16 keys");
   unsigned long long key[kKnn];
 #pragma unroll
   for (int s = 0; s < kKnn; ++s) key[s] = kNone;
-  int bx, by, bz;
-  if (floor_cell(xx, xy, xz, g, bx, by, bz)) {
-    for (int m = 0; m < 27; ++m) {
-      const int cell = stencil_cell(bx, by, bz, kNearFirst[m], g);
-      if (cell < 0) continue;
-      const int j1 = cs[cell + 1];
-      for (int j = cs[cell]; j < j1; ++j) {
-        const Packed3 pj = ld3(pos, j);
-        const float dx = pj.x - xx, dy = pj.y - xy, dz = pj.z - xz;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        const unsigned long long k = ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned)j;
-        // finite d2 only (NaN and inf never enter), and not the particle itself
-        if (!(d2 < __builtin_inff()) || j == i || !(k < key[kKnn - 1])) continue;
-        bool lt[kKnn];
-#pragma unroll
-        for (int s = 0; s < kKnn; ++s) lt[s] = k < key[s];
-#pragma unroll
-        for (int s = kKnn - 1; s > 0; --s) key[s] = lt[s - 1] ? key[s - 1] : (lt[s] ? k : key[s]);
-        key[0] = lt[0] ? k : key[0];
-      }
+  int bx = 0, by = 0, bz = 0;
+  const bool ok = floor_cell(xx, xy, xz, g, bx, by, bz);
+  const int z0 = bz - 1 > 0 ? bz - 1 : 0;
+  const int z1 = bz + 1 < g.cz - 1 ? bz + 1 : g.cz - 1;
+  // index range of row r of kRowNearFirst (empty outside the grid)
+  auto row_range = [&](int r, int &a, int &b) {
+    a = 0;
+    b = 0;
+    if (r >= 9) return;
+    const int rr = kRowNearFirst[r];
+    const int x = bx + rr / 3 - 1, y = by + rr % 3 - 1;
+    if (x >= 0 && x < g.cx && y >= 0 && y < g.cy) {
+      const int c0 = (x * g.cy + y) * g.cz;
+      a = cs[c0 + z0];
+      b = cs[c0 + z1 + 1];
     }
+  };
+  int r = ok && z0 <= z1 ? 0 : 9;  // row whose range is in (na, nb)
+  int na = 0, nb = 0;              // loaded one row ahead of use
+  row_range(r, na, nb);
+  int j = 0, j1 = 0;               // rest of the current row
+  for (;;) {
+    // the next 16 candidates (index -1 past the last row)
+    int jj[kKnn];
+#pragma unroll
+    for (int s = 0; s < kKnn; ++s) {
+      while (j >= j1 && r < 9) {
+        j = na;
+        j1 = nb;
+        row_range(++r, na, nb);
+      }
+      jj[s] = j < j1 ? j++ : -1;
+    }
+    unsigned long long c[kKnn];
+    bool need = false;
+#pragma unroll
+    for (int s = 0; s < kKnn; ++s) {
+      const int js = jj[s];
+      const Packed3 pj = ld3(pos, js >= 0 ? js : i);
+      const float dx = pj.x - xx, dy = pj.y - xy, dz = pj.z - xz;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      // finite d2 only (NaN and inf never enter), and not the particle itself
+      const bool take = js >= 0 && js != i && d2 < __builtin_inff();
+      c[s] = take ? ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned)js : kNone;
+      need |= c[s] < key[kKnn - 1];
+    }
+    if (__ballot(need)) {
+      kv_sort16(c);
+      kv_merge16(key, c);
+    }
+    if (!__ballot(j < j1 || r < 9)) break;
   }
   int kj[kKnn];
 #pragma unroll
