@@ -51,7 +51,7 @@ def upsample_phase(args, lib, torch, C, np):
     The learned upsampler runs on the same states: trained on the GPU on 50k
     pairs from 3 snapshots after the warm-up, then timed beside the plain
     one (features + network + teacher fallback on flagged particles) and
-    judged against the teacher at the end.  Its network roofline counts
+    scored against the teacher at the end.  Its network roofline counts
     2 (96 128 + 2 128 128 + 128 24) = 96,256 flops per coarse particle
     against the 2.5 PFLOP/s dense bf16 peak."""
     from granular_gpu_amd import _lib, scenes
@@ -294,7 +294,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU slab "
-                          "decomposition is a design (DESIGN.md), not built",
+                          "decomposition is designed, not built",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 

@@ -1,6 +1,6 @@
 // CPU oracle for the plain granular PBD step -- TEST INFRASTRUCTURE ONLY.
 // Restates the reference algorithm (stingyemperor/granular_gpu) with the
-// deterministic semantics documented in oracle.h / DESIGN.md.  Compiled with
+// deterministic semantics documented in oracle.h.  Compiled with
 // -ffp-contract=off and without -ffast-math: every FP32 operation below is an
 // individually rounded IEEE operation, in the order written.
 #include "oracle.h"

@@ -186,7 +186,7 @@ def test_inference_configs_agree(gpu):
 
 
 def test_training_gains_over_rigid_template(gpu):
-    # train on snapshots of one run, judge on another seed's dynamic phase
+    # train on snapshots of one run, score on another seed's dynamic phase
     from granular_gpu_amd.device import PlainSim
     from granular_gpu_amd.learned import LearnedUpsampler, MlpNet, Trainer, fine_error
     sim = PlainSim(scenes.plain_box(12, 1.0, r=R, dt=0.002, g=-9.8, seed=1))
