@@ -161,7 +161,10 @@ def upsample_phase(args, lib, torch, C, np):
         lib.gg_profiler_enable(0)
         ls = {name: 1000.0 * tot[k] / cnt[k] for k, name in enumerate(_lib.STAGES) if cnt[k]}
         t_net = round(ls["mlp"], 3)
-        flops = 96256 * n
+        # the network runs in as many launches per call as the call splits
+        # the particles into (two halves above 8192 particles)
+        per_call = cnt[_lib.STAGES.index("mlp")] / args.upsample_steps
+        flops = 96256 * n / per_call
         ach_l = flops / (t_net * 1e-6) / 1e12
         learned_out = {
             "network": "96-128-128-128-24 ReLU, bf16 weights/activations, fp32 accumulate",
@@ -169,7 +172,7 @@ def upsample_phase(args, lib, torch, C, np):
             "fine_particles_per_s": round(nf / (t_all * 1e-6), 1),
             "speedup_vs_plain": round((up_us + sort_us) / t_all, 3),
             "us_per_call": round(t_all, 3),
-            "network_us": t_net,
+            "network_us_per_launch": t_net, "network_launches_per_call": per_call,
             "fallback_coarse": nflag,
             "error_vs_teacher_r_fine": round(e_l, 5),
             "rigid_template_error_r_fine": round(e_r, 5),
@@ -177,10 +180,12 @@ def upsample_phase(args, lib, torch, C, np):
                          "achieved": round(ach_l, 2), "peak": 2500.0, "unit": "TFLOP/s",
                          "frac": round(ach_l / 2500.0, 4),
                          "traffic": traffic_of("void gg::k_mlp_infer<8, 4>"),
-                         "algorithmic_flops_per_launch": flops, "avg_launch_us": t_net},
+                         "algorithmic_flops_per_launch": int(flops), "avg_launch_us": t_net},
         }
         print(f"# learned: {1e-9 * nf / (t_all * 1e-6):.2f} G fine/s, error {e_l:.4f} r_fine "
               f"(rigid {e_r:.4f})", file=sys.stderr, flush=True)
+        if args.scale_side > 0:
+            learned_out["at_8m_fine"] = scale_phase(args, lib, torch, C, np, learned)
     return {
         "workload": f"{side}^3 coarse lattice r=0.02 (m=8) in a {box}^3 m box, {per} fine "
                     f"per coarse (offsets U(+-0.02), seed 7), {scene.cells[0]}^3 cells, "
@@ -196,6 +201,69 @@ def upsample_phase(args, lib, torch, C, np):
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic_of("gg::k_upsample"),
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_us": up_us},
         "learned": learned_out,
+    }
+
+
+def scale_phase(args, lib, torch, C, np, learned):
+    """The plain and the learned upsampler at 1M coarse particles (100^3
+    lattice, r = 0.02, 8 fine each: 8M fine), with the network trained on
+    the 250k scene.  Plain: fine sort + pass from the stage timers over the
+    timed steps; learned: whole calls between events on the calling stream."""
+    from granular_gpu_amd import _lib, scenes
+    from granular_gpu_amd.device import FineSet, PlainSim
+    from granular_gpu_amd.learned import fine_error
+
+    side, per = args.scale_side, 8
+    box = float(np.float32(0.04 * side + 0.52))
+    scene = scenes.plain_box(side, box, dt=args.dt, g=-9.8, r=0.02)
+    sim = PlainSim(scene, stab=3, solver=5)
+    fs = FineSet(sim, scenes.fine_cloud(scene.pos, per, 0.02, seed=7))
+    n, nf = sim.n, fs.n
+    print(f"# scale scene: coarse={n} fine={nf} cells={scene.cells}", file=sys.stderr,
+          flush=True)
+    for _ in range(args.upsample_warmup):
+        sim.step()
+        fs.sort()
+        fs.upsample()
+    steps = max(10, args.upsample_steps // 4)
+    torch.cuda.synchronize()
+    lib.gg_profiler_enable(steps * 32 + 16)
+    for _ in range(steps):
+        sim.step()
+        fs.sort()
+        fs.upsample()
+    torch.cuda.synchronize()
+    tot = (C.c_double * len(_lib.STAGES))()
+    cnt = (C.c_int * len(_lib.STAGES))()
+    _lib.check("gg_profiler_collect", lib.gg_profiler_collect(tot, cnt, len(_lib.STAGES)))
+    lib.gg_profiler_enable(0)
+    avg = {name: 1000.0 * tot[k] / cnt[k] for k, name in enumerate(_lib.STAGES) if cnt[k]}
+    plain_us = avg["upsample"] + avg["upsample_sort"]
+    lpos = torch.empty((n, 24), dtype=torch.float32, device=sim.pos.device)
+    learned.infer(sim, fine_pos=lpos)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        learned.infer(sim, fine_pos=lpos)
+    e1.record()
+    torch.cuda.synchronize()
+    t_all = 1000.0 * e0.elapsed_time(e1) / steps
+    _, tfp, _ = learned.teacher(sim)
+    err = fine_error(lpos, tfp) / 0.01
+    print(f"# scale: plain {plain_us:.1f} us, learned {t_all:.1f} us, error {err:.4f} r_fine",
+          file=sys.stderr, flush=True)
+    return {
+        "workload": f"{side}^3 coarse lattice r=0.02 in a {box:.2f}^3 m box, {per} fine per "
+                    f"coarse, {scene.cells[0]}^3 cells, steps {args.upsample_warmup}.."
+                    f"{args.upsample_warmup + steps}; network trained on the 250k scene",
+        "coarse": n, "fine": nf,
+        "plain_us": round(plain_us, 3),
+        "plain_fine_particles_per_s": round(nf / (plain_us * 1e-6), 1),
+        "learned_us_per_call": round(t_all, 3),
+        "learned_fine_particles_per_s": round(nf / (t_all * 1e-6), 1),
+        "speedup_vs_plain": round(plain_us / t_all, 3),
+        "error_vs_teacher_r_fine": round(err, 5),
     }
 
 
@@ -271,6 +339,8 @@ def main():
     ap.add_argument("--only", choices=["upsample", "batch"], default=None,
                     help="run one secondary phase alone and print its object (profiling)")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--scale-side", type=int, default=100,
+                    help="coarse lattice side of the 8M-fine comparison (0 skips it)")
     ap.add_argument("--learned-train-steps", type=int, default=2000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=64)

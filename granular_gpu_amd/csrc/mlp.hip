@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(256)
   int na = 0, nb = 0;              // loaded one row ahead of use
   row_range(r, na, nb);
   int j = 0, j1 = 0;               // rest of the current row
-  for (;;) {
+  for (bool first = true;; first = false) {
     // the next 16 candidates (index -1 past the last row)
     int jj[kKnn];
 #pragma unroll
@@ -250,7 +250,12 @@ __global__ void __launch_bounds__(256)
       c[s] = take ? ((unsigned long long)__float_as_uint(d2) << 32) | (unsigned)js : kNone;
       need |= c[s] < key[kKnn - 1];
     }
-    if (__ballot(need)) {
+    if (first) {
+      // nothing kept yet: the sorted chunk is the merge's result
+      kv_sort16(c);
+#pragma unroll
+      for (int s = 0; s < kKnn; ++s) key[s] = c[s];
+    } else if (__ballot(need)) {
       kv_sort16(c);
       kv_merge16(key, c);
     }
