@@ -157,13 +157,14 @@ def upsample_phase(args, lib, torch, C, np):
         for _ in range(args.upsample_steps):
             learned.infer(sim, fine_pos=lpos)
         torch.cuda.synchronize()
-        _lib.check("gg_profiler_collect", lib.gg_profiler_collect(tot, cnt, len(_lib.STAGES)))
+        # gg_profiler_collect adds to the arrays it is given: fresh ones
+        ltot = (C.c_double * len(_lib.STAGES))()
+        lcnt = (C.c_int * len(_lib.STAGES))()
+        _lib.check("gg_profiler_collect", lib.gg_profiler_collect(ltot, lcnt, len(_lib.STAGES)))
         lib.gg_profiler_enable(0)
-        ls = {name: 1000.0 * tot[k] / cnt[k] for k, name in enumerate(_lib.STAGES) if cnt[k]}
-        t_net = round(ls["mlp"], 3)
-        # the network runs in as many launches per call as the call splits
-        # the particles into (two halves above 8192 particles)
-        per_call = cnt[_lib.STAGES.index("mlp")] / args.upsample_steps
+        k_mlp = _lib.STAGES.index("mlp")
+        t_net = round(1000.0 * ltot[k_mlp] / lcnt[k_mlp], 3)
+        per_call = lcnt[k_mlp] / args.upsample_steps  # network launches per call
         flops = 96256 * n / per_call
         ach_l = flops / (t_net * 1e-6) / 1e12
         learned_out = {
