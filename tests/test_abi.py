@@ -55,6 +55,41 @@ def test_host_only_calls():
     assert lib.gg_workspace_init(C.byref(ws), None) == _lib.GG_ERR_INVALID
 
 
+def test_motion_and_extras_reject_bad_arguments_on_host():
+    # argument checks of the animation, explosion and picking entry points
+    # return before any device work
+    from granular_gpu_amd import _lib
+    lib = _lib.load()
+
+    class F3(C.Structure):
+        _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+    P = C.POINTER(_lib.gg_particles)
+    lib.gg_boundary_animate.argtypes = [P, F3, C.c_float, F3, C.c_void_p]
+    lib.gg_explosion_scratch_bytes.restype = C.c_size_t
+    lib.gg_explosion_scratch_bytes.argtypes = [C.c_int]
+    lib.gg_explosion.argtypes = [P, C.c_float, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
+    lib.gg_pick_scratch_bytes.restype = C.c_size_t
+    lib.gg_pick_scratch_bytes.argtypes = [C.c_int]
+    lib.gg_pick_sphere.argtypes = [P, F3, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_size_t, C.c_void_p]
+    lib.gg_move_picked.argtypes = [P, C.c_void_p, C.c_void_p, C.c_int, F3, F3, F3, C.c_void_p]
+    z = F3(0.0, 0.0, 0.0)
+    s = _lib.gg_particles()
+    assert lib.gg_boundary_animate(None, z, 10.0, z, None) == _lib.GG_ERR_INVALID
+    assert lib.gg_boundary_animate(C.byref(s), z, 10.0, z, None) == _lib.GG_OK  # n = 0
+    s.n = 5
+    assert lib.gg_boundary_animate(C.byref(s), z, 10.0, z, None) == _lib.GG_ERR_INVALID
+    assert lib.gg_explosion_scratch_bytes(1000) >= 4 * 8 * 4
+    assert lib.gg_explosion(C.byref(s), 50.0, None, 0, None, None) == _lib.GG_ERR_INVALID
+    s.pos = s.vel = s.mass = 1
+    assert lib.gg_explosion(C.byref(s), 50.0, 1, 0, None, None) == _lib.GG_ERR_WORKSPACE
+    assert lib.gg_pick_scratch_bytes(1000) >= 2 * 4 * 1000
+    assert lib.gg_pick_sphere(C.byref(s), z, 1.0, 1, None, 1, 1 << 20, None) == \
+        _lib.GG_ERR_INVALID
+    assert lib.gg_move_picked(C.byref(s), 1, 1, 6, z, z, z, None) == _lib.GG_ERR_INVALID
+    assert lib.gg_move_picked(C.byref(s), 1, 1, -1, z, z, z, None) == _lib.GG_ERR_INVALID
+
+
 def host_header_functions():
     txt = (ROOT / "include" / "granular_host.h").read_text()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
