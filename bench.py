@@ -57,7 +57,7 @@ def upsample_phase(args, lib, torch, C, np):
     from granular_gpu_amd import _lib, scenes
     from granular_gpu_amd.device import FineSet, PlainSim
 
-    side, box, per = args.upsample_side, 3.0, 8
+    side, box, per = args.upsample_side, args.upsample_box, 8
     scene = scenes.plain_box(side, box, dt=args.dt, g=-9.8, r=0.02)
     n, nb = scene.n, scene.boundary.shape[0]
     cells = scene.cells[0] * scene.cells[1] * scene.cells[2]
@@ -335,6 +335,7 @@ def main():
                     help="every n-th timed step carries per-stage hipEvents (0 = none)")
     ap.add_argument("--no-upsample", action="store_true")
     ap.add_argument("--upsample-side", type=int, default=63, help="63 -> 250,047 coarse")
+    ap.add_argument("--upsample-box", type=float, default=3.0)
     ap.add_argument("--upsample-steps", type=int, default=100)
     ap.add_argument("--upsample-warmup", type=int, default=20)
     ap.add_argument("--only", choices=["upsample", "batch"], default=None,
