@@ -212,6 +212,32 @@ def test_interactive_extras_match_oracle(gpu):
     sysm.close()
 
 
+def test_motion_and_explosion_at_full_size(gpu):
+    # a 400k-particle cone boundary (the funnel's size) rotated and shifted,
+    # and the explosion impulse on 1M particles, bit for bit with the oracle
+    from granular_gpu_amd import host
+    s = scenes.plain_box(100, 2.4, dt=0.002, g=-9.8)
+    cone = scenes.cone_funnel(400000, 30.0, 0.1, 0.6, (1.2, 0.9, 1.2))
+    boundary = np.concatenate([s.boundary, cone]).astype(np.float32)
+    anim = np.zeros(boundary.shape[0], np.int32)
+    anim[s.boundary.shape[0]:] = 1
+    _log(f"full size: n={s.n} boundary={boundary.shape[0]}")
+    sysm = host.HostSystem(s.pos, boundary, np.zeros((0, 3), np.float32), s.space,
+                           s.cell_length, s.dt, s.gravity, s.cells, s.density,
+                           is_move_boundary=True, boundary_animated=anim)
+    b, a = sysm.read("boundary", "pos"), sysm.read("boundary", "is_animated")
+    assert int(a.sum()) == 400000
+    sysm.animate((0.001, -0.002, 0.0), -1.5, (1.2, 0.9, 0.0))
+    assert np.array_equal(sysm.read("boundary", "pos"),
+                          po.animate(b, a, (0.001, -0.002, 0.0), -1.5, (1.2, 0.9)))
+    pos, vel = sysm.read("granular", "pos"), sysm.read("granular", "vel")
+    mass = sysm.read("granular", "mass")
+    com = sysm.explode(50.0)
+    assert (np.abs(com - po.center_of_mass(pos, mass)) <= np.spacing(np.abs(com))).all()
+    assert np.array_equal(sysm.read("granular", "vel"), po.explode(pos, vel, mass, com, 50.0))
+    sysm.close()
+
+
 def test_system_outputs(gpu, tmp_path):
     # VTK files and the checkpoint carry the system's own arrays
     from granular_gpu_amd import host
