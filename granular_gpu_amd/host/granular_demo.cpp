@@ -48,7 +48,8 @@ int main(int argc, char **argv) {
     // moved by "animate" is re-sorted every step as well
     const bool is_move = cfg.scene == EXCAVATOR || animate;
     std::cout << "granular " << granular_particles->size() << ", boundary "
-              << boundary_particles->size() << ", fine " << upsampled_particles->size()
+              << boundary_particles->size() << " (animated " << b.anim.num_particles
+              << "), fine " << upsampled_particles->size()
               << ", cells " << cfg.cell_size.x << "x" << cfg.cell_size.y << "x"
               << cfg.cell_size.z << "\n";
     auto p_system = std::make_shared<GranularSystem>(
