@@ -53,7 +53,8 @@ def upsample_phase(args, lib, torch, C, np):
     one (features + network + teacher fallback on flagged particles) and
     scored against the teacher at the end.  Its network roofline counts
     2 (96 128 + 2 128 128 + 128 24) = 96,256 flops per coarse particle
-    against the 2.5 PFLOP/s dense bf16 peak."""
+    against the 2.5 PFLOP/s dense bf16 peak.  With --scale-side > 0 the two
+    upsamplers are also compared at 8M fine particles (scale_phase)."""
     from granular_gpu_amd import _lib, scenes
     from granular_gpu_amd.device import FineSet, PlainSim
 
