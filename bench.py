@@ -293,7 +293,9 @@ def batch_phase(args, lib, torch, C, np):
     b.step_eager(ne)
     torch.cuda.synchronize()
     eager = (time.perf_counter() - t0) / ne
-    tune = b.autotune(trials=2)
+    # 5 timed launches per candidate (32 candidates, state restored between
+    # them): 2 left the choice to launch-to-launch noise
+    tune = b.autotune(trials=5)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     b.step(args.batch_steps)
