@@ -21,7 +21,9 @@ constexpr float kPiF = 3.14159265358979323846f;
 // that of every particle of its stencil stay below half of this fraction of
 // the skin.
 constexpr float kListValidFrac = 0.9f;
-constexpr int kFallbackWaveMax = 6;  // fallbacks per wave handed to whole waves (else per lane)
+// A wave with at most this many particles that must scan the full stencil
+// queues them for one wave each (else each scans in its own lane).
+constexpr int kFallbackWaveMax = 32;
 constexpr int kFallbackSlots = 62;  // ctr[1 + k % 62] = full-stencil count of sweep k
 // ctr[63..] are not used by the sweeps; the adaptive passes use ctr2 (own region)
 constexpr int kMergeRounds = 12;    // parallel claim-resolution rounds before the finisher
@@ -49,6 +51,8 @@ struct Layout {
   size_t meta;     // int4[cap]      neighbour list meta
   size_t glist;    // int[gmax*cap]  granular candidates, [k][i]
   size_t blist;    // int[bmax*cap]  boundary candidates, [k][i]
+  size_t fbq;      // int[cap]       particles a sweep queued for a whole-wave stencil scan
+  size_t fbqc;     // int[2]         queue lengths of the even / odd sweeps
   size_t ftmp0;    // float[cmax]    payload staging
   size_t itmp0;    // int[cmax]
   size_t itmp1;    // int[cmax]
@@ -113,6 +117,8 @@ inline Layout make_layout(int capacity, int bcap, gg_grid g, int max_sweeps,
   L.meta = take(cap * 16);
   L.glist = take((size_t)gmax * cap * 4);
   L.blist = take((size_t)bmax * cap * 4);
+  L.fbq = take(cap * 4);
+  L.fbqc = take(2 * 4);
   L.ftmp0 = take(cmax * 4);
   L.itmp0 = take(cmax * 4);
   L.itmp1 = take(cmax * 4);

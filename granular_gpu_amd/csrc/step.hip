@@ -22,7 +22,7 @@
 // displacement of any particle is below 0.9*skin, which guarantees every
 // pair outside the list is still out of contact.  Otherwise the particle
 // scans the full 27-cell stencil, in its own lane or, when few lanes of its
-// wave need to, by a whole wave after the list work of the block.  All paths
+// wave need to, queued and scanned by a whole wave in a second pass.  All paths
 // visit contacts in the same order, so the result is identical to always
 // scanning the stencil.
 #include <hip/hip_runtime.h>
@@ -46,10 +46,12 @@ __global__ void __launch_bounds__(kBlock)
                       int *__restrict__ rem_tmp, float *__restrict__ mer_tmp,
                       int *__restrict__ mc_tmp, float dt, float dvx, float dvy, float dvz,
                       double denom, int *__restrict__ dirty, int ndirty,
-                      int *__restrict__ fbcount, unsigned int *__restrict__ bpart) {
+                      int *__restrict__ fbcount, int *__restrict__ fbq_len,
+                      unsigned int *__restrict__ bpart) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   for (int k = i; k < ndirty; k += gridDim.x * blockDim.x) dirty[k] = 0;
   if (i < kFallbackSlots) fbcount[i] = 0;
+  if (i < 2) fbq_len[i] = 0;
   float dmax = 0.0f, rmax = 0.0f;
   if (i < n) {
     const int o = perm ? perm[i] : i;
@@ -207,6 +209,9 @@ struct SweepArgs {
   const float4 *__restrict__ d0;
   float4 *__restrict__ xb_out;
   int *__restrict__ stats;    // full-stencil particle count of this sweep
+  int *__restrict__ fbq;      // particles queued for a whole-wave stencil scan
+  int *__restrict__ fbq_count;  // queue length of this sweep
+  int *__restrict__ fbq_next;   // queue length of the next sweep (cleared by the queue pass)
   DevGrid g;
   SweepConst sc;
   int use_lists;
@@ -285,27 +290,37 @@ struct StencilRows {
   int start[9], len[9], total;
 };
 
-__device__ __forceinline__ void stencil_row_table(int bx, int by, int bz, const DevGrid &g,
-                                                  const int *__restrict__ cs, int lane,
-                                                  StencilRows &rt) {
-  int s = 0, e = 0;
+// The row tables of one stencil over the boundary (bcs) and granular (cs)
+// cell tables; lanes 0..8 read row r = lane of both, all loads in flight at
+// once.
+__device__ __forceinline__ void stencil_row_tables(int bx, int by, int bz, const DevGrid &g,
+                                                   const int *__restrict__ bcs,
+                                                   const int *__restrict__ cs, int lane,
+                                                   StencilRows &rb, StencilRows &rg) {
+  int sb = 0, eb = 0, sg = 0, eg = 0;
   const int z0 = bz - 1 > 0 ? bz - 1 : 0;
   const int z1 = bz + 1 < g.cz - 1 ? bz + 1 : g.cz - 1;
   if (lane < 9 && z0 <= z1) {
     const int x = bx + lane / 3 - 1, y = by + lane % 3 - 1;
     if (x >= 0 && x < g.cx && y >= 0 && y < g.cy) {
       const int c = (x * g.cy + y) * g.cz;
-      s = cs[c + z0];
-      e = cs[c + z1 + 1];
+      sb = bcs[c + z0];
+      eb = bcs[c + z1 + 1];
+      sg = cs[c + z0];
+      eg = cs[c + z1 + 1];
     }
   }
-  const int len = e - s;
-  rt.total = 0;
+  const int lb = eb - sb, lg = eg - sg;
+  rb.total = 0;
+  rg.total = 0;
 #pragma unroll
   for (int r = 0; r < 9; ++r) {
-    rt.start[r] = __builtin_amdgcn_readlane(s, r);
-    rt.len[r] = __builtin_amdgcn_readlane(len, r);
-    rt.total += rt.len[r];
+    rb.start[r] = __builtin_amdgcn_readlane(sb, r);
+    rb.len[r] = __builtin_amdgcn_readlane(lb, r);
+    rb.total += rb.len[r];
+    rg.start[r] = __builtin_amdgcn_readlane(sg, r);
+    rg.len[r] = __builtin_amdgcn_readlane(lg, r);
+    rg.total += rg.len[r];
   }
 }
 
@@ -351,23 +366,42 @@ __device__ __forceinline__ void wave_stencil(const SweepArgs &a, int i, const fl
                                              int &ng) {
   const SweepConst &sc = a.sc;
   const int lane = threadIdx.x & 63;
-  StencilRows rt;
-  stencil_row_table(bx, by, bz, a.g, a.bcs, lane, rt);
-  for (int t0 = 0; t0 < rt.total; t0 += 64) {
-    const int k = stencil_row_index(rt, t0 + lane);
+  StencilRows rb, rg;
+  stencil_row_tables(bx, by, bz, a.g, a.bcs, a.cs, lane, rb, rg);
+  // The first 2 x 64 boundary and 4 x 64 granular candidates are loaded
+  // before any is used (one memory latency instead of one per 64).
+  const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int kq[2], jq[4];
+  float4 q[2], gb[4], go[4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    kq[u] = stencil_row_index(rb, 64 * u + lane);
+    q[u] = kq[u] >= 0 ? a.bnd4[kq[u]] : z4;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    jq[u] = stencil_row_index(rg, 64 * u + lane);
+    gb[u] = jq[u] >= 0 ? a.base[jq[u]] : z4;
+    go[u] = jq[u] >= 0 ? a.off_in[jq[u]] : z4;
+  }
+  for (int t0 = 0; t0 < rb.total; t0 += 64) {
+    const int u = t0 >> 6;
+    const int k = u < 2 ? kq[u & 1] : stencil_row_index(rb, t0 + lane);
     float3 c = make_float3(0.0f, 0.0f, 0.0f), f = c;
     bool hit = false;
-    if (k >= 0) hit = boundary_terms<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, c, f);
+    if (k >= 0) hit = boundary_terms<FRIC>(bi, oi, u < 2 ? q[u & 1] : a.bnd4[k], rcb, preb, sc, c, f);
     const unsigned long long m = __ballot(hit);
     nb += __popcll(m);
     add_lanes<FRIC>(m, c, f, ab);
   }
-  stencil_row_table(bx, by, bz, a.g, a.cs, lane, rt);
-  for (int t0 = 0; t0 < rt.total; t0 += 64) {
-    const int j = stencil_row_index(rt, t0 + lane);
+  for (int t0 = 0; t0 < rg.total; t0 += 64) {
+    const int u = t0 >> 6;
+    const int j = u < 4 ? jq[u & 3] : stencil_row_index(rg, t0 + lane);
     float3 c = make_float3(0.0f, 0.0f, 0.0f), f = c;
     bool hit = false;
-    if (j >= 0 && j != i) hit = granular_terms<FRIC>(bi, oi, a.base[j], a.off_in[j], sc, c, f);
+    if (j >= 0 && j != i)
+      hit = granular_terms<FRIC>(bi, oi, u < 4 ? gb[u & 3] : a.base[j],
+                                 u < 4 ? go[u & 3] : a.off_in[j], sc, c, f);
     const unsigned long long m = __ballot(hit);
     ng += 2 * __popcll(m);
     add_lanes<FRIC>(m, c, f, ag);
@@ -376,18 +410,12 @@ __device__ __forceinline__ void wave_stencil(const SweepArgs &a, int i, const fl
 
 // One sweep.  Particles whose candidate list is provably exact for this
 // sweep use it; the others scan the full stencil, either in their own lane
-// or (when few lanes of the wave need it) queued in LDS and scanned by a
-// whole wave of the block after the list work.  All paths visit contacts in
+// or (when few lanes of the wave need it) queued for k_sweep_queue, which
+// scans each with a whole wave.  All paths visit contacts in
 // the same order, so the result does not depend on which path ran.
 template <bool FRIC, bool LAST_STAB, int BS>
 __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
-  __shared__ int queue[BS];
-  __shared__ int queued;
   const int lane = threadIdx.x & 63;
-  if (a.use_lists) {
-    if (threadIdx.x == 0) queued = 0;
-    __syncthreads();
-  }
   const int i = xcd_block(blockIdx.x, gridDim.x, a.remap) * blockDim.x + threadIdx.x;
   const DevGrid &g = a.g;
   const SweepConst &sc = a.sc;
@@ -433,14 +461,22 @@ __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
     // count particles that scanned the full stencil (one atomic per wave)
     atomicAdd(a.stats, (int)__popcll(fb));
   }
+  // Stencil scans of a few lanes go to the sweep's queue (one wave each,
+  // spread over the grid by k_sweep_queue): scanned in their own lane they
+  // would hold this wave several times longer than its list work.  A slot
+  // past the capacity (a queue that was not drained) scans in its lane.
   const unsigned long long scan = __ballot(mode == 2);
-  const bool to_queue = a.use_lists && scan && __popcll(scan) <= a.fb_wave_max;
-  if (to_queue) {
+  bool queued = false;
+  if (a.use_lists && scan && __popcll(scan) <= a.fb_wave_max) {
     const int leader = __ffsll((long long)scan) - 1;
     int at0 = 0;
-    if (lane == leader) at0 = atomicAdd(&queued, (int)__popcll(scan));
+    if (lane == leader) at0 = atomicAdd(a.fbq_count, (int)__popcll(scan));
     at0 = __shfl(at0, leader, 64);
-    if (mode == 2) queue[at0 + __popcll(scan & ((1ull << lane) - 1ull))] = i;
+    const int at = at0 + __popcll(scan & ((1ull << lane) - 1ull));
+    if (mode == 2 && at < sc.cap) {
+      a.fbq[at] = i;
+      queued = true;
+    }
   }
   const float rcb = sc.rb + fmaxf(bi.w, sc.rmin);
   const float preb = contact_prefilter(rcb);
@@ -469,21 +505,39 @@ __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
       const int j = gl[t * stride];
       granular_contact<FRIC>(bi, oi, a.base[j], a.off_in[j], sc, ag, ng);
     }
-  } else if (mode == 2 && !to_queue) {
+  } else if (mode == 2 && !queued) {
     full_stencil<FRIC>(a, i, bi, oi, bx, by, bz, rcb, preb, ab, ag, nb, ng);
   }
-  if (mode == 1 || mode == 3 || (mode == 2 && !to_queue))
+  if (mode == 1 || mode == 3 || (mode == 2 && !queued))
     sweep_store<LAST_STAB>(a, i, bi, oi, ab, ag, nb + ng, mt.w);
-  if (!a.use_lists) return;
-  // queued particles: one wave each
-  __syncthreads();
-  const int nq = queued;
-  for (int q = threadIdx.x >> 6; q < nq; q += BS / 64) {
-    const int iq = queue[q];
+}
+
+// The particles k_sweep queued: one wave per particle, the waves of the
+// whole grid taking them in turn, so that the few particles whose lists
+// lapsed cost the sweep about one stencil scan of latency.  Same inputs and
+// outputs as k_sweep (a Jacobi sweep reads off_in only).  Sweeps alternate
+// between two queue lengths; this pass clears the next sweep's, which no
+// kernel reads or writes meanwhile (predict clears both at the start of a
+// step).  A queue that was not cleared (a sweep repeated without its
+// neighbours) only makes particles scan again: the full-stencil result is
+// every particle's exact result, so rewriting it changes nothing.
+template <bool FRIC, bool LAST_STAB>
+__global__ void __launch_bounds__(256) k_sweep_queue(SweepArgs a) {
+  const int lane = threadIdx.x & 63;
+  const DevGrid &g = a.g;
+  const SweepConst &sc = a.sc;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.fbq_next = 0;
+  const int len = __atomic_load_n(a.fbq_count, __ATOMIC_RELAXED);
+  const int nq = len < sc.cap ? len : sc.cap;
+  const int waves = gridDim.x * (blockDim.x >> 6);
+  for (int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nq; q += waves) {
+    const int iq = a.fbq[q];
+    if (iq < 0 || iq >= a.n) continue;
     const float4 bq = a.base[iq];
     const float4 oq = a.off_in[iq];
     int qx = 0, qy = 0, qz = 0;
-    trunc_cell(bq.x + oq.x, bq.y + oq.y, bq.z + oq.z, g, qx, qy, qz);
+    // (only a stale entry can fail this; k_sweep stored its result)
+    if (!trunc_cell(bq.x + oq.x, bq.y + oq.y, bq.z + oq.z, g, qx, qy, qz)) continue;
     const float rq = sc.rb + fmaxf(bq.w, sc.rmin);
     float3 qb = make_float3(0.0f, 0.0f, 0.0f), qg = qb;
     int nqb = 0, nqg = 0;
@@ -607,7 +661,7 @@ static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, cons
       set->particle_2_cell, ad ? ad->remove : nullptr, ad ? ad->merge : nullptr,
       ad ? ad->merge_count : nullptr, at<int>(ws, L.itmp0), at<float>(ws, L.ftmp1),
       at<int>(ws, L.itmp1), p->dt, dvx, dvy, dvz, denom, at<int>(ws, L.dirty), ndirty,
-      at<int>(ws, L.ctr) + 1, at<unsigned int>(ws, L.bpart));
+      at<int>(ws, L.ctr) + 1, at<int>(ws, L.fbqc), at<unsigned int>(ws, L.bpart));
   GG_LAUNCH_CHECK();
   k_reduce_bounds<<<1, 1024, 0, st>>>(at<unsigned int>(ws, L.bpart), nblk,
                                       at<unsigned int>(ws, L.bounds));
@@ -635,6 +689,12 @@ static int launch_sweep(const SweepArgs &a, int bs, hipStream_t st) {
     default: k_sweep<FRIC, LAST, 256><<<(n + 255) / 256, 256, 0, st>>>(a); break;
   }
   GG_LAUNCH_CHECK();
+  if (a.use_lists) {
+    // up to 2048 waves take the queued particles in turn
+    const int qblocks = n / 256 + 1 < 512 ? n / 256 + 1 : 512;
+    k_sweep_queue<FRIC, LAST><<<qblocks, 256, 0, st>>>(a);
+    GG_LAUNCH_CHECK();
+  }
   return GG_OK;
 }
 
@@ -662,6 +722,9 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   a.d0 = at<float4>(ws, L.d0);
   a.xb_out = at<float4>(ws, L.xb);
   a.stats = at<int>(ws, L.ctr) + 1 + (k % kFallbackSlots);
+  a.fbq = at<int>(ws, L.fbq);
+  a.fbq_count = at<int>(ws, L.fbqc) + (k & 1);
+  a.fbq_next = at<int>(ws, L.fbqc) + ((k + 1) & 1);
   a.g = dev_grid(ws->grid);
   a.sc = sweep_const(p, ws);
   a.use_lists = lists_on(p, ws) ? 1 : 0;
