@@ -12,8 +12,10 @@ neighbour lists + 13 contact sweeps + finalize, inputs resident in HBM.
 Prints one JSON line.  `value` = particle-steps/s over the timed steps.
 Per-stage times come from hipEvents on the launch stream around every stage
 of every 10th timed step (--stage-every); the other steps carry no events.
-`roofline` covers the dominant kernel (the contact sweep) from those events;
-its algorithmic bytes per launch are
+`roofline` covers the dominant kernel (the solver contact sweep) from those
+events; each sweep is followed by a short pass over the particles it queued
+for whole-wave stencil scans, timed as its own stage (`sweep_queue`).  The
+sweep's algorithmic bytes per launch are
 N*32 + N_b*12 (read x*, m, m_s and write x* per particle; boundary positions
 once).  `cpu_baseline` runs the C++/OpenMP FP32 oracle on a bounded sample
 of the same state.
@@ -415,7 +417,7 @@ def main():
     every = 0 if args.no_stage_timing else max(0, args.stage_every)
     sampled = [k for k in range(args.steps) if every and k % every == 0]
     if sampled:
-        lib.gg_profiler_enable(len(sampled) * (K + 4) + 16)
+        lib.gg_profiler_enable(len(sampled) * (2 * K + 4) + 16)  # sweeps + queue passes
         lib.gg_profiler_pause(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

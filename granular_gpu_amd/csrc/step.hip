@@ -689,12 +689,15 @@ static int launch_sweep(const SweepArgs &a, int bs, hipStream_t st) {
     default: k_sweep<FRIC, LAST, 256><<<(n + 255) / 256, 256, 0, st>>>(a); break;
   }
   GG_LAUNCH_CHECK();
-  if (a.use_lists) {
-    // up to 2048 waves take the queued particles in turn
-    const int qblocks = n / 256 + 1 < 512 ? n / 256 + 1 : 512;
-    k_sweep_queue<FRIC, LAST><<<qblocks, 256, 0, st>>>(a);
-    GG_LAUNCH_CHECK();
-  }
+  return GG_OK;
+}
+
+template <bool FRIC, bool LAST>
+static int launch_sweep_queue(const SweepArgs &a, hipStream_t st) {
+  // up to 2048 waves take the queued particles in turn
+  const int qblocks = a.n / 256 + 1 < 512 ? a.n / 256 + 1 : 512;
+  k_sweep_queue<FRIC, LAST><<<qblocks, 256, 0, st>>>(a);
+  GG_LAUNCH_CHECK();
   return GG_OK;
 }
 
@@ -746,6 +749,16 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
     rc = launch_sweep<false, false>(a, bs, st);
   if (rc != GG_OK) return rc;
   prof_end(stage, st);
+  if (!a.use_lists) return GG_OK;  // nothing is queued without lists
+  prof_begin(GG_STAGE_SWEEP_QUEUE, st);
+  if (last_stab)
+    rc = launch_sweep_queue<false, true>(a, st);
+  else if (fric)
+    rc = launch_sweep_queue<true, false>(a, st);
+  else
+    rc = launch_sweep_queue<false, false>(a, st);
+  if (rc != GG_OK) return rc;
+  prof_end(GG_STAGE_SWEEP_QUEUE, st);
   return GG_OK;
 }
 

@@ -288,7 +288,9 @@ enum {
   GG_STAGE_MLP = 9,      /* learned upsampler: fused MFMA network */
   GG_STAGE_MLP_FEATURES = 10, /* learned upsampler: kNN-16 features  */
   GG_STAGE_MLP_FALLBACK = 11, /* learned upsampler: teacher fallback */
-  GG_STAGE_COUNT = 12
+  GG_STAGE_SWEEP_QUEUE = 12,  /* one pass over the particles a contact sweep
+                                 queued for whole-wave stencil scans */
+  GG_STAGE_COUNT = 13
 };
 int gg_profiler_enable(int max_records); /* 0 disables */
 /* Nonzero stops recording new stages (records so far are kept), 0 resumes;
