@@ -212,6 +212,7 @@ struct SweepArgs {
   int *__restrict__ fbq;      // particles queued for a whole-wave stencil scan
   int *__restrict__ fbq_count;  // queue length of this sweep
   int *__restrict__ fbq_next;   // queue length of the next sweep (cleared by the queue pass)
+  int fbq_global;  // 1: queue in fbq for k_sweep_queue, 0: in the block's LDS queue
   DevGrid g;
   SweepConst sc;
   int use_lists;
@@ -408,14 +409,54 @@ __device__ __forceinline__ void wave_stencil(const SweepArgs &a, int i, const fl
   }
 }
 
+// wave_stencil without the hoisted loads (each group of 64 candidates is
+// loaded when it is used): the variant k_sweep inlines, where registers set
+// the occupancy of the list work.
+template <bool FRIC>
+__device__ __forceinline__ void wave_stencil_seq(const SweepArgs &a, int i, const float4 &bi,
+                                                 const float4 &oi, int bx, int by, int bz,
+                                                 float rcb, float preb, float3 &ab, float3 &ag,
+                                                 int &nb, int &ng) {
+  const SweepConst &sc = a.sc;
+  const int lane = threadIdx.x & 63;
+  StencilRows rb, rg;
+  stencil_row_tables(bx, by, bz, a.g, a.bcs, a.cs, lane, rb, rg);
+  for (int t0 = 0; t0 < rb.total; t0 += 64) {
+    const int k = stencil_row_index(rb, t0 + lane);
+    float3 c = make_float3(0.0f, 0.0f, 0.0f), f = c;
+    bool hit = false;
+    if (k >= 0) hit = boundary_terms<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, c, f);
+    const unsigned long long m = __ballot(hit);
+    nb += __popcll(m);
+    add_lanes<FRIC>(m, c, f, ab);
+  }
+  for (int t0 = 0; t0 < rg.total; t0 += 64) {
+    const int j = stencil_row_index(rg, t0 + lane);
+    float3 c = make_float3(0.0f, 0.0f, 0.0f), f = c;
+    bool hit = false;
+    if (j >= 0 && j != i) hit = granular_terms<FRIC>(bi, oi, a.base[j], a.off_in[j], sc, c, f);
+    const unsigned long long m = __ballot(hit);
+    ng += 2 * __popcll(m);
+    add_lanes<FRIC>(m, c, f, ag);
+  }
+}
+
+
 // One sweep.  Particles whose candidate list is provably exact for this
 // sweep use it; the others scan the full stencil, either in their own lane
 // or (when few lanes of the wave need it) queued for k_sweep_queue, which
 // scans each with a whole wave.  All paths visit contacts in
 // the same order, so the result does not depend on which path ran.
 template <bool FRIC, bool LAST_STAB, int BS>
-__global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
+__global__ void __launch_bounds__(BS, 6) k_sweep(SweepArgs a) {
+  __shared__ int lqueue[BS];
+  __shared__ int lqueued;
   const int lane = threadIdx.x & 63;
+  const bool local_q = a.use_lists && !a.fbq_global;
+  if (local_q) {
+    if (threadIdx.x == 0) lqueued = 0;
+    __syncthreads();
+  }
   const int i = xcd_block(blockIdx.x, gridDim.x, a.remap) * blockDim.x + threadIdx.x;
   const DevGrid &g = a.g;
   const SweepConst &sc = a.sc;
@@ -461,21 +502,32 @@ __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
     // count particles that scanned the full stencil (one atomic per wave)
     atomicAdd(a.stats, (int)__popcll(fb));
   }
-  // Stencil scans of a few lanes go to the sweep's queue (one wave each,
-  // spread over the grid by k_sweep_queue): scanned in their own lane they
-  // would hold this wave several times longer than its list work.  A slot
+  // Stencil scans of a few lanes are queued (one wave each): scanned in
+  // their own lane they would hold this wave several times longer than its
+  // list work.  Large sets queue sweep-wide for k_sweep_queue, which spreads
+  // them over the grid; small sets queue in the block (LDS, at most BS
+  // entries), scanned by the block's waves at the end.  A sweep-wide slot
   // past the capacity (a queue that was not drained) scans in its lane.
   const unsigned long long scan = __ballot(mode == 2);
   bool queued = false;
   if (a.use_lists && scan && __popcll(scan) <= a.fb_wave_max) {
     const int leader = __ffsll((long long)scan) - 1;
+    const int rank = __popcll(scan & ((1ull << lane) - 1ull));
     int at0 = 0;
-    if (lane == leader) at0 = atomicAdd(a.fbq_count, (int)__popcll(scan));
-    at0 = __shfl(at0, leader, 64);
-    const int at = at0 + __popcll(scan & ((1ull << lane) - 1ull));
-    if (mode == 2 && at < sc.cap) {
-      a.fbq[at] = i;
-      queued = true;
+    if (a.fbq_global) {
+      if (lane == leader) at0 = atomicAdd(a.fbq_count, (int)__popcll(scan));
+      at0 = __shfl(at0, leader, 64);
+      if (mode == 2 && at0 + rank < sc.cap) {
+        a.fbq[at0 + rank] = i;
+        queued = true;
+      }
+    } else {
+      if (lane == leader) at0 = atomicAdd(&lqueued, (int)__popcll(scan));
+      at0 = __shfl(at0, leader, 64);
+      if (mode == 2) {
+        lqueue[at0 + rank] = i;
+        queued = true;
+      }
     }
   }
   const float rcb = sc.rb + fmaxf(bi.w, sc.rmin);
@@ -510,6 +562,23 @@ __global__ void __launch_bounds__(BS) k_sweep(SweepArgs a) {
   }
   if (mode == 1 || mode == 3 || (mode == 2 && !queued))
     sweep_store<LAST_STAB>(a, i, bi, oi, ab, ag, nb + ng, mt.w);
+  if (!local_q) return;
+  // the block's queued particles: one wave each
+  __syncthreads();
+  const int nq = lqueued;
+  for (int q = threadIdx.x >> 6; q < nq; q += BS / 64) {
+    const int iq = lqueue[q];
+    const float4 bq = a.base[iq];
+    const float4 oq = a.off_in[iq];
+    int qx = 0, qy = 0, qz = 0;
+    trunc_cell(bq.x + oq.x, bq.y + oq.y, bq.z + oq.z, g, qx, qy, qz);
+    const float rq = sc.rb + fmaxf(bq.w, sc.rmin);
+    float3 qb = make_float3(0.0f, 0.0f, 0.0f), qg = qb;
+    int nqb = 0, nqg = 0;
+    wave_stencil_seq<FRIC>(a, iq, bq, oq, qx, qy, qz, rq, contact_prefilter(rq), qb, qg, nqb,
+                           nqg);
+    if (lane == 0) sweep_store<LAST_STAB>(a, iq, bq, oq, qb, qg, nqb + nqg, a.meta[iq].w);
+  }
 }
 
 // The particles k_sweep queued: one wave per particle, the waves of the
@@ -734,9 +803,13 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   a.remap = (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1;
   a.bounds = at<unsigned int>(ws, L.bounds);
   a.near_limit = 0.999f * ws->grid.cell_length;
+  // GG_FLAG_FALLBACK_WAVE queues every stencil scan sweep-wide, whatever the
+  // size of the set
+  a.fbq_global = (n >= kSweepQueueMinParticles || (p->flags & GG_FLAG_FALLBACK_WAVE)) ? 1 : 0;
   a.fb_wave_max = (p->flags & GG_FLAG_FALLBACK_LANE)   ? 0
                   : (p->flags & GG_FLAG_FALLBACK_WAVE) ? 64
-                                                       : kFallbackWaveMax;
+                  : a.fbq_global                       ? kFallbackWaveMax
+                                                       : kFallbackWaveMaxLocal;
   const int bs = p->sweep_block == 0 ? 64 : p->sweep_block;
   const int stage = k < S ? GG_STAGE_STAB : GG_STAGE_SWEEP;
   prof_begin(stage, st);
@@ -749,7 +822,8 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
     rc = launch_sweep<false, false>(a, bs, st);
   if (rc != GG_OK) return rc;
   prof_end(stage, st);
-  if (!a.use_lists) return GG_OK;  // nothing is queued without lists
+  // nothing is queued sweep-wide without lists, per block, or in lanes only
+  if (!a.use_lists || !a.fbq_global || a.fb_wave_max == 0) return GG_OK;
   prof_begin(GG_STAGE_SWEEP_QUEUE, st);
   if (last_stab)
     rc = launch_sweep_queue<false, true>(a, st);
