@@ -148,7 +148,9 @@ def test_neighbour_lists_bitexact(gpu):
 
 def test_fallback_paths_agree(gpu):
     # particles whose list is not valid for a sweep scan the stencil in their
-    # own lane or by a whole wave; both must give the oracle's result
+    # own lane or by a whole wave: "auto" queues them in the block (this set
+    # is small), "wave" forces the sweep-wide queue and its second pass, "lane"
+    # scans every one in its lane; all must give the oracle's result
     import torch
     from granular_gpu_amd import _lib
     steps = 60
