@@ -80,9 +80,11 @@ typedef struct gg_params {
 #define GG_FLAG_NO_NLIST 1     /* always scan the 27-cell stencil */
 #define GG_FLAG_NO_XCD_REMAP 2 /* keep hardware block order (no XCD grouping) */
 /* Particles whose neighbour list is not valid for a sweep scan the 27-cell
- * stencil instead: one wave per particle when few lanes of a wave need it,
- * one lane per particle otherwise.  These force one way; results are the
- * same bit for bit. */
+ * stencil instead: one wave per particle when few lanes of a wave need it
+ * (queued in the block for sets below 2^19 particles, else sweep-wide and
+ * scanned by a second pass after the sweep), one lane per particle
+ * otherwise.  These force one way (WAVE: every such particle queued
+ * sweep-wide, whatever the size); results are the same bit for bit. */
 #define GG_FLAG_FALLBACK_LANE 4
 #define GG_FLAG_FALLBACK_WAVE 8
 
