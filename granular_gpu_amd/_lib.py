@@ -27,6 +27,10 @@ STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep", "adapti
           "upsample_sort", "mlp", "mlp_features", "mlp_fallback", "sweep_queue"]
 
 
+class gg_float3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
 class gg_grid(C.Structure):
     _fields_ = [("cell_length", C.c_float), ("cx", C.c_int), ("cy", C.c_int), ("cz", C.c_int)]
 
@@ -115,6 +119,10 @@ _PROTOS = {
                                    P(gg_workspace), VP]),
     "gg_finalize": (C.c_int, [P(gg_particles), P(gg_params), P(gg_workspace), VP]),
     "gg_workspace_view": (VP, [P(gg_workspace), C.c_int]),
+    "gg_add_external_force": (C.c_int, [P(gg_particles), gg_float3, VP]),
+    "gg_predict_positions": (C.c_int, [P(gg_particles), C.c_float, VP, VP]),
+    "gg_mass_scaling": (C.c_int, [P(gg_particles), VP]),
+    "gg_sort_positions_by_key": (C.c_int, [P(gg_particles), VP, P(gg_workspace), VP]),
     "gg_surface": (C.c_int, [P(gg_particles), VP, P(gg_params), P(gg_workspace), VP]),
     "gg_surface_distance": (C.c_int, [P(gg_particles), VP, P(gg_workspace), VP]),
     "gg_merge_mark": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, C.c_float, P(gg_workspace),

@@ -16,8 +16,10 @@
 
 namespace gg {
 
-// key_i = hash(x_i); slot_i = arrival order inside the cell.
-__global__ void __launch_bounds__(kBlock) k_hash_count(const float *__restrict__ pos, int n,
+// key_i = hash(x_i) (or keys_in[i] when given: out-of-range keys go to the
+// sentinel cell); slot_i = arrival order inside the cell.
+__global__ void __launch_bounds__(kBlock) k_hash_count(const float *__restrict__ pos,
+                                                       const int *__restrict__ keys_in, int n,
                                                        DevGrid g, int *__restrict__ key,
                                                        int *__restrict__ slot,
                                                        int *__restrict__ count,
@@ -27,8 +29,14 @@ __global__ void __launch_bounds__(kBlock) k_hash_count(const float *__restrict__
   // All lanes take part in the wave operations; lanes past n carry key -1.
   int c = -1;
   if (i < n) {
-    const Packed3 p3 = ld3(pos, i);
-    c = hash_cell(p3.x, p3.y, p3.z, g);
+    if (keys_in) {
+      const int C = g.cx * g.cy * g.cz;
+      c = keys_in[i];
+      if (c < 0 || c > C) c = C;
+    } else {
+      const Packed3 p3 = ld3(pos, i);
+      c = hash_cell(p3.x, p3.y, p3.z, g);
+    }
     key[i] = c;
   }
   // Wave-aggregated slot reservation: each run of equal keys in adjacent
@@ -300,9 +308,10 @@ __global__ void __launch_bounds__(kBlock) k_pack_boundary(const float *__restric
 
 static inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
-// Hash, counting sort, cell table and perm for `n` positions.
-int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *ws,
-                 hipStream_t st) {
+// Hash (or the given keys), counting sort, cell table and perm for `n`
+// particles.
+int sort_by_cell_keys(const float *pos, const int *keys_in, int n, int *cell_start,
+                      const gg_workspace *ws, hipStream_t st) {
   Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
                          ws->bmax);
   const DevGrid g = dev_grid(ws->grid);
@@ -315,7 +324,7 @@ int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *w
   int *big = at<int>(ws, L.big);
   int *ctr = at<int>(ws, L.ctr);
   const int nb = n > 0 ? blocks_for(n) : 1;
-  k_hash_count<<<nb, kBlock, 0, st>>>(pos, n, g, key, slot, count, ctr);
+  k_hash_count<<<nb, kBlock, 0, st>>>(pos, keys_in, n, g, key, slot, count, ctr);
   GG_LAUNCH_CHECK();
   const int np = (m + kScanTile - 1) / kScanTile;
   k_scan_reduce<<<np, kBlock, 0, st>>>(count, m, part);
@@ -336,6 +345,11 @@ int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *w
   return GG_OK;
 }
 
+int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *ws,
+                 hipStream_t st) {
+  return sort_by_cell_keys(pos, nullptr, n, cell_start, ws, st);
+}
+
 // out[k] = sum of in[0..k) for k in [0, m], in[] cleared.  m <= kScanTile *
 // kScanMaxParts (the workspace's partials array holds the tile sums).
 int scan_exclusive(int *in, int m, int *out, const gg_workspace *ws, hipStream_t st) {
@@ -354,8 +368,7 @@ int scan_exclusive(int *in, int m, int *out, const gg_workspace *ws, hipStream_t
   return GG_OK;
 }
 
-static int gather_inplace(void *arr, int words, const int *perm, int n, void *tmp,
-                          hipStream_t st) {
+int gather_inplace(void *arr, int words, const int *perm, int n, void *tmp, hipStream_t st) {
   if (!arr || n <= 0) return GG_OK;
   k_gather_words<<<blocks_for(n), kBlock, 0, st>>>(static_cast<const uint32_t *>(arr),
                                                    static_cast<uint32_t *>(tmp), perm, n, words);

@@ -7,6 +7,11 @@ namespace gg {
 // the workspace's key / perm arrays.
 int sort_by_cell(const float *pos, int n, int *cell_start, const gg_workspace *ws,
                  hipStream_t st);
+// The same sort by given cell keys (keys outside [0, C] count as C).
+int sort_by_cell_keys(const float *pos, const int *keys_in, int n, int *cell_start,
+                      const gg_workspace *ws, hipStream_t st);
+// arr[k] = arr[perm[k]] for n records of `words` 32-bit words, through tmp.
+int gather_inplace(void *arr, int words, const int *perm, int n, void *tmp, hipStream_t st);
 // sort_by_cell + in-place permutation of the set's payload arrays.
 int grid_build_general(gg_particles *set, gg_adaptive *ad, int *cell_start, int *perm_out,
                        const gg_workspace *ws, hipStream_t st, int capacity_limit);

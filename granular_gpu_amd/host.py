@@ -88,6 +88,7 @@ def load() -> C.CDLL:
         "ggh_system_set_solver": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float,
                                             C.c_int]),
         "ggh_darray_selftest": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
+        "ggh_solver_stages_selftest": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
         "ggh_write_vtk": (C.c_int, [C.c_char_p, C.c_char_p, vp, C.c_int]),
         "ggh_system_save_upsampled_vtk": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int]),
         "ggh_system_save_boundary_vtk": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int]),
@@ -184,6 +185,17 @@ def darray_selftest(n: int) -> str:
     msg = buf.value.decode()
     if rc != 0:
         raise HostError(f"DArray self-test (n={n}) failed: {msg}")
+    return msg
+
+
+def solver_stages_selftest(n: int) -> str:
+    """C++ self-test of the Solver's separate stage methods on n particles."""
+    lib = load()
+    buf = C.create_string_buffer(512)
+    rc = lib.ggh_solver_stages_selftest(int(n), buf, 512)
+    msg = buf.value.decode()
+    if rc != 0:
+        raise HostError(f"Solver stage self-test (n={n}) failed: {msg}")
     return msg
 
 

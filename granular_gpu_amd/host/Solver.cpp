@@ -300,18 +300,51 @@ void Solver::upsampled_update(std::shared_ptr<GranularParticles> &particles,
         "gg_upsample_plain");
 }
 
-float3 Solver::trigger_explosion(std::shared_ptr<GranularParticles> &particles,
-                                 float explosion_force) {
-  float3 com = make_float3(0.0f, 0.0f, 0.0f);
+void Solver::trigger_explosion(std::shared_ptr<GranularParticles> &particles,
+                               float explosion_force) {
+  _explosion_center = make_float3(0.0f, 0.0f, 0.0f);
   const int n = (int)particles->size();
-  if (n == 0) return com;
+  if (n == 0) return;
   DArray<unsigned int> scratch((unsigned int)((gg_explosion_scratch_bytes(n) + 3) / 4));
   gg_particles s = particles->view();
   gg_float3 c{0.0f, 0.0f, 0.0f};
   check(gg_explosion(&s, explosion_force, scratch.addr(), 4 * (size_t)scratch.length(), &c,
                      nullptr),
         "gg_explosion");
-  return make_float3(c.x, c.y, c.z);
+  _explosion_center = make_float3(c.x, c.y, c.z);
+}
+
+void Solver::add_external_force(std::shared_ptr<GranularParticles> &particles, float dt,
+                                float3 G) {
+  gg_particles s = particles->view();
+  const gg_float3 dv{dt * G.x, dt * G.y, dt * G.z};
+  check(gg_add_external_force(&s, dv, nullptr), "gg_add_external_force");
+}
+
+void Solver::update_particle_positions(std::shared_ptr<GranularParticles> &particles, float dt) {
+  const unsigned int n = particles->size();
+  if (!_pos_t || _pos_t->capacity() < n) _pos_t.reset(new DArray<float3>(n));
+  _pos_t->set_length(n);
+  gg_particles s = particles->view();
+  check(gg_predict_positions(&s, dt, reinterpret_cast<gg_float3 *>(_pos_t->addr()), nullptr),
+        "gg_predict_positions");
+}
+
+void Solver::apply_mass_scaling(std::shared_ptr<GranularParticles> &particles) {
+  gg_particles s = particles->view();
+  check(gg_mass_scaling(&s, nullptr), "gg_mass_scaling");
+}
+
+void Solver::update_neighborhood(const std::shared_ptr<GranularParticles> &particles) {
+  if (!_ws_ready)
+    throw std::runtime_error("update_neighborhood: the grid is set by the first sort or step");
+  if ((int)particles->size() > _ws.capacity)
+    ensure_workspace((int)particles->size(), _ws.boundary_capacity, _ws.grid);
+  const unsigned int cells = (unsigned int)((long long)_ws.grid.cx * _ws.grid.cy * _ws.grid.cz + 2);
+  if (!_key_cells || _key_cells->length() != cells) _key_cells.reset(new DArray<int>(cells));
+  gg_particles s = particles->view();
+  check(gg_sort_positions_by_key(&s, _key_cells->addr(), &_ws, nullptr),
+        "gg_sort_positions_by_key");
 }
 
 void Solver::save_particle_stats(const std::string &filename) {

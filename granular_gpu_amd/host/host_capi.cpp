@@ -351,7 +351,8 @@ int ggh_system_explode(ggh_system *h, float force, float com_out[3]) {
   try {
     if (!h) throw std::invalid_argument("null handle");
     auto p = h->sys->get_particles_non_const();
-    const float3 c = h->sys->get_solver().trigger_explosion(p, force);
+    h->sys->get_solver().trigger_explosion(p, force);
+    const float3 c = h->sys->get_solver().get_explosion_center();
     if (com_out) {
       com_out[0] = c.x;
       com_out[1] = c.y;

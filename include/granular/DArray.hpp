@@ -1,26 +1,62 @@
-// Owning device array: the host DArray<T> of the reference (DArray.hpp:12-257)
-// over hipMalloc and the library's compaction primitive.
+// Owning device array of the host API.  Keeps the member functions and
+// signatures of the reference's DArray<T> (DArray.hpp:12-257: length(),
+// capacity(), addr(), compact(flags), append(other), remove(start, count),
+// resize(capacity), verifyArrayOrdering(...)) so that code written against it
+// compiles unchanged; the storage scheme is this library's own:
 //
-// Same members and semantics: length()/capacity()/addr(); compact(flags)
-// keeps the elements with flag != 1 in order and throws on a length
-// mismatch; append() grows to max(new length, 2 x capacity) when full;
-// remove(start, count) closes the gap; resize(new_capacity) reallocates and
-// keeps min(length, capacity) elements; verifyArrayOrdering() compacts a set
-// of arrays and checks on the host that survivors kept their order.
+// - Every array owns two device blocks of `capacity` elements, the live one
+//   and a spare.  compact() and remove() write the surviving elements into
+//   the spare (the library's ordered compaction, gg_compact) and then swap
+//   the two, so neither allocates nor copies device-to-device per call.
+// - compact() also keeps a scratch block for the compaction's per-block
+//   counts, grown with the capacity; the kept count is read back once.
+// - append() grows the capacity geometrically (by half, plus a 64-element
+//   floor) when the new length does not fit.
+// - Device errors throw std::runtime_error naming the operation and the HIP
+//   error string.
 // Elements must be a whole number of 32-bit words (float, int, float3, ...).
 #pragma once
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <cstring>
-#include <iostream>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <typeinfo>
+#include <utility>
 #include <vector>
 
 #include "../granular_gpu.h"
 #include "Global.hpp"
+
+namespace granular_detail {
+
+inline void hip_ok(hipError_t e, const char *op) {
+  if (e != hipSuccess)
+    throw std::runtime_error(std::string("DArray ") + op + ": " + hipGetErrorString(e));
+}
+
+// Zero-filled device block of `count` elements of `elem` bytes (at least one).
+inline void *device_block(size_t count, size_t elem, const char *op) {
+  const size_t n = std::max<size_t>(count, 1);
+  if (n > std::numeric_limits<size_t>::max() / elem)
+    throw std::runtime_error(std::string("DArray ") + op + ": size overflows");
+  void *p = nullptr;
+  hip_ok(hipMalloc(&p, n * elem), op);
+  const hipError_t e = hipMemset(p, 0, n * elem);
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    hip_ok(e, op);
+  }
+  return p;
+}
+
+inline void release(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+}  // namespace granular_detail
 
 template <typename T>
 class DArray {
@@ -29,173 +65,191 @@ class DArray {
  public:
   explicit DArray(unsigned int size, unsigned int initial_capacity = 0)
       : _addr(nullptr), _length(size), _capacity(std::max(size, initial_capacity)) {
-    if (size > 1000000000u) throw std::runtime_error("Suspiciously large array size requested");
-    const size_t bytes = sizeof(T) * (size_t)std::max(_capacity, 1u);
-    size_t free_mem = 0, total_mem = 0;
-    if (hipMemGetInfo(&free_mem, &total_mem) == hipSuccess && bytes > free_mem) {
-      std::cerr << "Not enough GPU memory. Required: " << bytes << " Available: " << free_mem
-                << std::endl;
-      throw std::runtime_error("Insufficient GPU memory");
-    }
-    if (hipMalloc(reinterpret_cast<void **>(&_addr), bytes) != hipSuccess) {
-      _addr = nullptr;
-      throw std::runtime_error("HIP malloc failed");
-    }
-    if (hipMemset(_addr, 0, bytes) != hipSuccess) {
-      (void)hipFree(_addr);
-      _addr = nullptr;
-      throw std::runtime_error("HIP memset failed");
-    }
+    _addr = static_cast<T *>(granular_detail::device_block(_capacity, sizeof(T), "allocate"));
   }
 
   DArray(const DArray &) = delete;
   DArray &operator=(const DArray &) = delete;
 
   ~DArray() {
-    if (_addr) {
-      if (hipFree(_addr) != hipSuccess)
-        std::cerr << "Failed to free HIP memory in destructor" << std::endl;
-      _addr = nullptr;
-    }
+    granular_detail::release(_addr);
+    granular_detail::release(_spare);
+    granular_detail::release(_scratch);
   }
 
   unsigned int length() const { return _length; }
   unsigned int capacity() const { return _capacity; }
   T *addr() const { return _addr; }
 
-  // Stable removal of the elements whose flag is 1 (0 and -1 are kept).
+  // Keeps the elements whose flag is not 1, in their order; returns the new
+  // length.  The flags must cover exactly the current length.
   unsigned int compact(const DArray<int> &removal_flags) {
-    if (removal_flags.length() != _length) {
-      std::cerr << "Array type: " << typeid(T).name() << "\nRemoval flags length ("
-                << removal_flags.length() << ") doesn't match array length (" << _length << ")"
-                << std::endl;
-      throw std::runtime_error("Removal flags length mismatch");
-    }
+    if (removal_flags.length() != _length)
+      throw std::runtime_error(std::string("DArray<") + typeid(T).name() +
+                               ">::compact: " + std::to_string(removal_flags.length()) +
+                               " flags for " + std::to_string(_length) + " elements");
     if (_length == 0) return 0;
-    const size_t sbytes = gg_compact_scratch_bytes(_length);
-    void *tmp = nullptr, *scratch = nullptr;
-    if (hipMalloc(&tmp, sizeof(T) * (size_t)_length) != hipSuccess)
-      throw std::runtime_error("HIP malloc failed in compact");
-    if (hipMalloc(&scratch, sbytes + 256) != hipSuccess) {
-      (void)hipFree(tmp);
-      throw std::runtime_error("HIP malloc failed in compact");
+    ensure_spare();
+    const size_t need = gg_compact_scratch_bytes(_length) + sizeof(int);
+    if (need > _scratch_bytes) {
+      granular_detail::release(_scratch);
+      _scratch = nullptr;
+      _scratch_bytes = 0;
+      const size_t grow = gg_compact_scratch_bytes(_capacity) + sizeof(int);
+      _scratch = granular_detail::device_block(std::max(need, grow), 1, "compact scratch");
+      _scratch_bytes = std::max(need, grow);
     }
-    int *d_count = reinterpret_cast<int *>(static_cast<char *>(scratch) + sbytes);
-    const int rc = gg_compact(_addr, tmp, _length, (int)(sizeof(T) / 4), removal_flags.addr(),
-                              d_count, scratch, sbytes, nullptr);
+    int *kept_dev = reinterpret_cast<int *>(static_cast<char *>(_scratch) + _scratch_bytes -
+                                            sizeof(int));
+    const int rc = gg_compact(_addr, _spare, _length, (int)(sizeof(T) / 4), removal_flags.addr(),
+                              kept_dev, _scratch, _scratch_bytes - sizeof(int), nullptr);
+    if (rc != GG_OK) throw std::runtime_error(std::string("DArray compact: ") + gg_status_string(rc));
     int kept = 0;
-    hipError_t e = hipSuccess;
-    if (rc == GG_OK) e = hipMemcpy(&kept, d_count, sizeof(int), hipMemcpyDeviceToHost);
-    if (rc == GG_OK && e == hipSuccess && kept > 0)
-      e = hipMemcpy(_addr, tmp, sizeof(T) * (size_t)kept, hipMemcpyDeviceToDevice);
-    (void)hipFree(tmp);
-    (void)hipFree(scratch);
-    if (rc != GG_OK) throw std::runtime_error(std::string("compact: ") + gg_status_string(rc));
-    if (e != hipSuccess) throw std::runtime_error("Compact copy failed");
+    granular_detail::hip_ok(hipMemcpy(&kept, kept_dev, sizeof(int), hipMemcpyDeviceToHost),
+                            "compact count");
+    std::swap(_addr, _spare);
     _length = (unsigned int)kept;
     return _length;
   }
 
+  // Appends the elements of other after the current ones.
   void append(const DArray<T> &other) {
-    const unsigned int other_length = other.length();
-    if (other_length == 0) return;
-    const unsigned int new_length = _length + other_length;
-    if (new_length > _capacity) resize(std::max(new_length, _capacity * 2));
-    if (hipMemcpy(_addr + _length, other.addr(), sizeof(T) * (size_t)other_length,
-                  hipMemcpyDeviceToDevice) != hipSuccess)
-      throw std::runtime_error("Failed to append array");
-    _length = new_length;
-    if (hipDeviceSynchronize() != hipSuccess)
-      throw std::runtime_error("Error occurred during array append");
+    const unsigned int extra = other.length();
+    if (extra == 0) return;
+    const unsigned long long want = (unsigned long long)_length + extra;
+    if (want > std::numeric_limits<unsigned int>::max())
+      throw std::runtime_error("DArray append: length overflows");
+    if (want > _capacity) {
+      const unsigned long long grown = (unsigned long long)_capacity + _capacity / 2 + 64;
+      resize_keep((unsigned int)std::min<unsigned long long>(
+          std::max(want, grown), std::numeric_limits<unsigned int>::max()));
+    }
+    granular_detail::hip_ok(hipMemcpy(_addr + _length, other.addr(), sizeof(T) * (size_t)extra,
+                                      hipMemcpyDeviceToDevice),
+                            "append");
+    _length = (unsigned int)want;
   }
 
+  // Drops elements [start, start + count) and closes the gap.
   void remove(unsigned int start, unsigned int count) {
     if ((unsigned long long)start + count > _length)
-      throw std::runtime_error("Remove range exceeds array bounds");
+      throw std::runtime_error("DArray remove: range [" + std::to_string(start) + ", " +
+                               std::to_string((unsigned long long)start + count) +
+                               ") is not inside the " + std::to_string(_length) + " elements");
+    if (count == 0) return;
     const unsigned int tail = _length - start - count;
-    if (count > 0 && tail > 0) {
-      T *tmp = nullptr;
-      if (hipMalloc(reinterpret_cast<void **>(&tmp), sizeof(T) * (size_t)tail) != hipSuccess)
-        throw std::runtime_error("HIP malloc failed in remove");
-      hipError_t e = hipMemcpy(tmp, _addr + start + count, sizeof(T) * (size_t)tail,
-                               hipMemcpyDeviceToDevice);
-      if (e == hipSuccess)
-        e = hipMemcpy(_addr + start, tmp, sizeof(T) * (size_t)tail, hipMemcpyDeviceToDevice);
-      (void)hipFree(tmp);
-      if (e != hipSuccess) throw std::runtime_error("Remove copy failed");
-    }
+    ensure_spare();
+    if (start > 0)
+      granular_detail::hip_ok(
+          hipMemcpy(_spare, _addr, sizeof(T) * (size_t)start, hipMemcpyDeviceToDevice), "remove");
+    if (tail > 0)
+      granular_detail::hip_ok(hipMemcpy(_spare + start, _addr + start + count,
+                                        sizeof(T) * (size_t)tail, hipMemcpyDeviceToDevice),
+                              "remove");
+    std::swap(_addr, _spare);
     _length -= count;
   }
 
+  // Compacts every array by the same flags and checks on the host that each
+  // survivor i holds the value of the i-th unflagged element from before.
+  // names (optional) label the arrays in the error text.
   static void verifyArrayOrdering(const std::vector<DArray<T> *> &arrays,
                                   const DArray<int> &removal_flags, const char *arrayNames[]) {
     if (arrays.empty()) return;
-    const unsigned int length = arrays[0]->length();
-    for (size_t i = 1; i < arrays.size(); i++)
-      if (arrays[i]->length() != length)
-        throw std::runtime_error(std::string("Array length mismatch between ") + arrayNames[0] +
-                                 " and " + arrayNames[i]);
-    std::vector<std::vector<T>> before(arrays.size()), after(arrays.size());
-    std::vector<int> flags(length);
-    for (size_t i = 0; i < arrays.size(); i++) {
-      before[i].resize(length);
-      HIP_CALL(hipMemcpy(before[i].data(), arrays[i]->addr(), sizeof(T) * (size_t)length,
-                         hipMemcpyDeviceToHost));
+    auto label = [&](size_t a) {
+      return arrayNames && arrayNames[a] ? std::string(arrayNames[a])
+                                         : "array " + std::to_string(a);
+    };
+    const unsigned int n = arrays[0]->length();
+    std::vector<int> flags(n);
+    if (n > 0)
+      granular_detail::hip_ok(hipMemcpy(flags.data(), removal_flags.addr(), sizeof(int) * (size_t)n,
+                                        hipMemcpyDeviceToHost),
+                              "verify flags");
+    // expected survivors, computed on the host from the snapshots
+    std::vector<std::vector<T>> expect(arrays.size());
+    for (size_t a = 0; a < arrays.size(); ++a) {
+      if (arrays[a]->length() != n)
+        throw std::runtime_error("verifyArrayOrdering: " + label(a) + " has " +
+                                 std::to_string(arrays[a]->length()) + " elements, " + label(0) +
+                                 " has " + std::to_string(n));
+      std::vector<T> snap(n);
+      if (n > 0)
+        granular_detail::hip_ok(hipMemcpy(snap.data(), arrays[a]->addr(), sizeof(T) * (size_t)n,
+                                          hipMemcpyDeviceToHost),
+                                "verify snapshot");
+      for (unsigned int i = 0; i < n; ++i)
+        if (flags[i] != 1) expect[a].push_back(snap[i]);
     }
-    HIP_CALL(hipMemcpy(flags.data(), removal_flags.addr(), sizeof(int) * (size_t)length,
-                       hipMemcpyDeviceToHost));
-    for (auto *a : arrays) a->compact(removal_flags);
-    const unsigned int new_length = arrays[0]->length();
-    for (size_t i = 0; i < arrays.size(); i++) {
-      if (arrays[i]->length() != new_length)
-        throw std::runtime_error("Inconsistent lengths after compaction");
-      after[i].resize(new_length);
-      HIP_CALL(hipMemcpy(after[i].data(), arrays[i]->addr(), sizeof(T) * (size_t)new_length,
-                         hipMemcpyDeviceToHost));
+    for (auto *arr : arrays) arr->compact(removal_flags);
+    for (size_t a = 0; a < arrays.size(); ++a) {
+      const unsigned int m = arrays[a]->length();
+      if (m != expect[a].size())
+        throw std::runtime_error("verifyArrayOrdering: " + label(a) + " kept " +
+                                 std::to_string(m) + " elements, expected " +
+                                 std::to_string(expect[a].size()));
+      std::vector<T> got(m);
+      if (m > 0)
+        granular_detail::hip_ok(hipMemcpy(got.data(), arrays[a]->addr(), sizeof(T) * (size_t)m,
+                                          hipMemcpyDeviceToHost),
+                                "verify result");
+      for (unsigned int i = 0; i < m; ++i)
+        if (std::memcmp(&got[i], &expect[a][i], sizeof(T)) != 0)
+          throw std::runtime_error("verifyArrayOrdering: " + label(a) + " survivor " +
+                                   std::to_string(i) + " is not the " + std::to_string(i) +
+                                   "-th unflagged element");
     }
-    std::vector<unsigned int> survivors;
-    for (unsigned int i = 0; i < length; i++)
-      if (flags[i] != 1) survivors.push_back(i);
-    if (survivors.size() != new_length)
-      throw std::runtime_error("Array ordering not maintained during compaction");
-    for (unsigned int i = 0; i < new_length; i++)
-      for (size_t a = 0; a < arrays.size(); a++)
-        if (std::memcmp(&after[a][i], &before[a][survivors[i]], sizeof(T)) != 0) {
-          std::cerr << "Order mismatch in " << arrayNames[a] << " at index " << i
-                    << " (original index " << survivors[i] << ")" << std::endl;
-          throw std::runtime_error("Array ordering not maintained during compaction");
-        }
   }
 
+  // New capacity; the first min(length, new_capacity) elements are kept.
   void resize(unsigned int new_capacity) {
-    T *new_addr = nullptr;
-    if (hipMalloc(reinterpret_cast<void **>(&new_addr),
-                  sizeof(T) * (size_t)std::max(new_capacity, 1u)) != hipSuccess)
-      throw std::runtime_error("HIP malloc failed in resize");
     const unsigned int keep = std::min(_length, new_capacity);
-    if (keep > 0)
-      HIP_CALL(hipMemcpy(new_addr, _addr, sizeof(T) * (size_t)keep, hipMemcpyDeviceToDevice));
-    (void)hipFree(_addr);
-    _addr = new_addr;
-    _capacity = new_capacity;
     _length = keep;
+    resize_keep(new_capacity);
   }
 
   // Sets the length after a device pass changed the element count in place
   // (merge/split compaction works inside the capacity).
   void set_length(unsigned int n) {
-    if (n > _capacity) throw std::runtime_error("set_length beyond capacity");
+    if (n > _capacity)
+      throw std::runtime_error("DArray set_length: " + std::to_string(n) + " exceeds capacity " +
+                               std::to_string(_capacity));
     _length = n;
   }
 
   // Grows the capacity to at least cap, keeping length and contents.
   void reserve(unsigned int cap) {
-    if (cap > _capacity) {
-      const unsigned int len = _length;
-      resize(cap);
-      _length = len;
-    }
+    if (cap > _capacity) resize_keep(cap);
   }
+
+ private:
+  // Reallocates both blocks at new_capacity (>= _length), keeping the
+  // first _length elements.
+  void resize_keep(unsigned int new_capacity) {
+    T *fresh = static_cast<T *>(granular_detail::device_block(new_capacity, sizeof(T), "resize"));
+    if (_length > 0) {
+      const hipError_t e =
+          hipMemcpy(fresh, _addr, sizeof(T) * (size_t)_length, hipMemcpyDeviceToDevice);
+      if (e != hipSuccess) {
+        granular_detail::release(fresh);
+        granular_detail::hip_ok(e, "resize copy");
+      }
+    }
+    granular_detail::release(_addr);
+    granular_detail::release(_spare);
+    _spare = nullptr;
+    _addr = fresh;
+    _capacity = new_capacity;
+  }
+
+  void ensure_spare() {
+    if (!_spare)
+      _spare = static_cast<T *>(granular_detail::device_block(_capacity, sizeof(T), "spare"));
+  }
+
+  T *_spare = nullptr;
+  void *_scratch = nullptr;
+  size_t _scratch_bytes = 0;
 
  public:
   T *_addr;

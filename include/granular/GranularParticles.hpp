@@ -7,7 +7,9 @@
 // add_elements appends mass, adaptive_last_step = 1, pos and vel, as in the
 // reference; the other arrays keep their constructed length.
 #pragma once
+#include <iostream>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "DArray.hpp"
@@ -39,31 +41,29 @@ class GranularParticles final : public Particles {
   int *get_is_animated_ptr() { return _is_animated.addr(); }
   const DArray<float> &get_mass() const { return _mass; }
 
+  // mass and adaptive_last_step report a flag-length mismatch on stderr and
+  // stay as they are; pos and vel (Particles::remove_elements) throw it.
   void remove_elements(const DArray<int> &removal_flags) {
-    try {
-      _mass.compact(removal_flags);
-    } catch (std::exception &e) {
-      std::cerr << "Mismatch in mass: " << e.what() << std::endl;
-    }
-    try {
-      _adaptive_last_step.compact(removal_flags);
-    } catch (std::exception &e) {
-      std::cerr << "Mismatch in adaptive_last_step: " << e.what() << std::endl;
-    }
+    compact_or_report(_mass, removal_flags, "mass");
+    compact_or_report(_adaptive_last_step, removal_flags, "adaptive_last_step");
     Particles::remove_elements(removal_flags);
   }
 
   void add_elements(const DArray<float> &mass, const DArray<float3> &pos,
                     const DArray<float3> &vel, const int num) {
     if ((int)mass.length() != num || (int)pos.length() != num || (int)vel.length() != num)
-      throw std::runtime_error("Input array sizes don't match count in add_elements");
-    const unsigned int new_size = size() + (unsigned int)num;
-    if (new_size > _mass.capacity()) _mass.resize(new_size);
+      throw std::runtime_error("GranularParticles::add_elements: count " + std::to_string(num) +
+                               " but " + std::to_string(mass.length()) + " masses, " +
+                               std::to_string(pos.length()) + " positions, " +
+                               std::to_string(vel.length()) + " velocities");
+    _mass.reserve(_mass.length() + (unsigned int)num);
     _mass.append(mass);
     DArray<int> ones((unsigned int)num);
     if (num > 0) {
       const int rc = gg_fill_u32(ones.addr(), (size_t)num, 1u, nullptr);
-      if (rc != GG_OK) throw std::runtime_error("fill failed in add_elements");
+      if (rc != GG_OK)
+        throw std::runtime_error(std::string("GranularParticles::add_elements: flag fill: ") +
+                                 gg_status_string(rc));
     }
     _adaptive_last_step.append(ones);
     Particles::add_elements(pos, vel);
@@ -125,6 +125,16 @@ class GranularParticles final : public Particles {
   virtual ~GranularParticles() noexcept {}
 
  protected:
+  template <typename T>
+  static void compact_or_report(DArray<T> &a, const DArray<int> &flags, const char *name) {
+    try {
+      a.compact(flags);
+    } catch (const std::exception &e) {
+      std::cerr << "GranularParticles::remove_elements left " << name << " unchanged: "
+                << e.what() << std::endl;
+    }
+  }
+
   DArray<float> _mass;
   DArray<float> _scaled_mass;
   DArray<int> _surface;

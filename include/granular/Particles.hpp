@@ -2,6 +2,7 @@
 // reference): built from host positions, velocities start at zero.
 #pragma once
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "DArray.hpp"
@@ -31,16 +32,15 @@ class Particles {
 
   void add_elements(const DArray<float3> &pos, const DArray<float3> &vel) {
     if (pos.length() != vel.length())
-      throw std::runtime_error("Position and velocity arrays must have same length");
-    const unsigned int new_size = _pos.length() + pos.length();
-    if (new_size > _pos.capacity()) {
-      _pos.resize(new_size);
-      _vel.resize(new_size);
-    }
+      throw std::runtime_error("Particles::add_elements: " + std::to_string(pos.length()) +
+                               " positions but " + std::to_string(vel.length()) + " velocities");
+    _pos.reserve(_pos.length() + pos.length());
+    _vel.reserve(_vel.length() + vel.length());
     _pos.append(pos);
     _vel.append(vel);
     if (_pos.length() != _vel.length())
-      throw std::runtime_error("Position and velocity array sizes mismatch after append");
+      throw std::runtime_error("Particles::add_elements: position and velocity lengths differ "
+                               "after the append");
   }
 
   virtual ~Particles() noexcept {}

@@ -2,9 +2,11 @@
 // library's stage entry points.
 //
 // Differences a caller can see, each a design decision:
-// - add_external_force, update_particle_positions and apply_mass_scaling
-//   are one fused pass, predict() (gg_predict); separate passes would add
-//   two full read/write sweeps of the particle arrays.
+// - step() and the system's step run add_external_force,
+//   update_particle_positions and apply_mass_scaling as one fused pass,
+//   predict() (gg_predict); separate passes would add two full read/write
+//   sweeps of the particle arrays.  The reference's separate methods are
+//   kept as entry points of their own for callers that drive the stages.
 // - final_update also clears adaptive_last_step (the reference clears it in
 //   step() right after final_update).
 // - The merge-skip toggle is per instance (the reference's t_merge_iter is a
@@ -67,6 +69,16 @@ class Solver {
 
   void final_update(std::shared_ptr<GranularParticles> &particles, float dt);
 
+  // The reference's separate stages (Solver.cu:112-191), one pass each:
+  // v += dt G; predicted positions x + dt v into the solver's _pos_t
+  // buffer (get_pos_t_ptr); scaled mass m exp(-y); and the stable sort of
+  // the positions alone by particle_2_cell (the keys stay as they are).
+  void update_neighborhood(const std::shared_ptr<GranularParticles> &particles);
+  void add_external_force(std::shared_ptr<GranularParticles> &particles, float dt, float3 G);
+  void update_particle_positions(std::shared_ptr<GranularParticles> &particles, float dt);
+  void apply_mass_scaling(std::shared_ptr<GranularParticles> &particles);
+  float3 *get_pos_t_ptr() const { return _pos_t ? _pos_t->addr() : nullptr; }
+
   // Merge (skipped on the first call), blend, split, compaction
   // (Solver.cu:856-1242); N changes and the stats tuple is recorded.
   void adaptive_sampling(std::shared_ptr<GranularParticles> &particles,
@@ -106,8 +118,10 @@ class Solver {
 
   // Solver::trigger_explosion (Solver.cu:1411-1442): radial impulse
   // force / (1 + d^2) / m away from the centre of mass (gg_explosion; the
-  // centre is summed in double on the device).  Returns the centre.
-  float3 trigger_explosion(std::shared_ptr<GranularParticles> &particles, float explosion_force);
+  // centre is summed in double on the device).  The centre of the last
+  // explosion is kept for get_explosion_center().
+  void trigger_explosion(std::shared_ptr<GranularParticles> &particles, float explosion_force);
+  float3 get_explosion_center() const { return _explosion_center; }
 
   void save_particle_stats(const std::string &filename);
   const std::vector<std::tuple<int, int, int>> &get_particle_stats() const {
@@ -158,4 +172,9 @@ class Solver {
   float3 _last_G = make_float3(0.0f, -9.8f, 0.0f);
   float _last_density = 238732.4146f;
   std::vector<std::tuple<int, int, int>> _particle_stats;
+  // predicted positions of update_particle_positions, and the cell table
+  // that update_neighborhood's sort fills
+  std::unique_ptr<DArray<float3>> _pos_t;
+  std::unique_ptr<DArray<int>> _key_cells;
+  float3 _explosion_center = make_float3(0.0f, 0.0f, 0.0f);
 };

@@ -271,6 +271,26 @@ int gg_contact_sweep(gg_particles *set, const int *cell_start,
 int gg_finalize(gg_particles *set, const gg_params *params,
                 const gg_workspace *ws, hipStream_t stream);
 
+/* The reference Solver's separate per-particle stages (gg_step / gg_predict
+ * run them fused into one pass):
+ * gg_add_external_force: v += dv for the set's n particles; dv = dt * G
+ *   formed by the caller (Solver::add_external_force, Solver.cu:125-131).
+ * gg_predict_positions: out[i] = x_i + dt * v_i per component, into a
+ *   caller array of n float3 (update_particle_positions, Solver.cu:133-160,
+ *   which writes the solver's _pos_t).
+ * gg_mass_scaling: scaled_mass[i] = mass[i] * exp(-x_i.y) of the current
+ *   positions, exp in double rounded once (apply_mass_scaling,
+ *   Solver.cu:162-191).
+ * gg_sort_positions_by_key: stable sort of pos alone by particle_2_cell
+ *   (the keys themselves unchanged; keys outside [0, C] sort as C); the
+ *   cell table of those keys goes to cell_start (C+2)
+ *   (update_neighborhood, Solver.cu:112-123). */
+int gg_add_external_force(gg_particles *set, gg_float3 dv, hipStream_t stream);
+int gg_predict_positions(const gg_particles *set, float dt, gg_float3 *out, hipStream_t stream);
+int gg_mass_scaling(gg_particles *set, hipStream_t stream);
+int gg_sort_positions_by_key(gg_particles *set, int *cell_start, const gg_workspace *ws,
+                             hipStream_t stream);
+
 /* ---- stage profiler ------------------------------------------------------ */
 /* When enabled, every stage launched by gg_step and the other entry points
  * is bracketed by hipEvents on its stream (the reference times only the whole

@@ -164,6 +164,11 @@ int ggh_system_set_solver(ggh_system *h, int stab, int solver, float mu_s, float
  * verifyArrayOrdering) and GranularParticles (remove_elements,
  * add_elements) on n elements; 0 when every check passes. */
 int ggh_darray_selftest(int n, char *msg, int msg_len);
+/* Device self-test of the Solver's separate stages (add_external_force,
+ * update_particle_positions, apply_mass_scaling, update_neighborhood,
+ * trigger_explosion; Solver.cu:112-191, 1411-1442) on n particles against
+ * host-computed values; 0 and "ok" in msg on success. */
+int ggh_solver_stages_selftest(int n, char *msg, int msg_len);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,15 @@ def test_darray_selftest(gpu, n):
     assert host.darray_selftest(n) == "ok"
 
 
+@pytest.mark.parametrize("n", [0, 1, 257, 5000])
+def test_solver_stage_methods(gpu, n):
+    # add_external_force, update_particle_positions, apply_mass_scaling,
+    # update_neighborhood and trigger_explosion called as Solver methods
+    from granular_gpu_amd import host
+    _log(f"Solver stage self-test n={n}")
+    assert host.solver_stages_selftest(n) == "ok"
+
+
 def _unsorted_keys(keys_sorted, perm):
     k = np.empty_like(keys_sorted)
     k[perm] = keys_sorted

@@ -259,3 +259,56 @@ def test_m1_full_size_properties(gpu):
     dt, g = float(s.dt), float(s.gravity[1])
     drop = dt * dt * g * 3 * 4 / 2
     assert abs(pos[:, 1].mean() - (s.pos[:, 1].mean() + drop)) < 1e-5
+
+
+def test_separate_stages_equal_fused_predict(gpu):
+    # gg_add_external_force + gg_predict_positions + gg_mass_scaling equal
+    # the fused predict pass bit for bit, and each equals its IEEE float32
+    # formula evaluated by numpy; gg_sort_positions_by_key = numpy's stable
+    # argsort of the keys.
+    import ctypes as C
+
+    import torch
+    from granular_gpu_amd import _lib
+    s = scenes.plain_box(12, 1.0, dt=0.002, g=-9.8)
+    _log(f"separate stages n={s.n}")
+    sim = _sim(s, stab=3, solver=5)
+    sim.grid_build()
+    sim.predict()
+    xs = sim.view(5, s.n)[:, :3].cpu().numpy()
+    ms_fused = sim.scaled_mass[:s.n].cpu().numpy().copy()
+    pos0, vel0, mass0 = sim.host()
+    lib = sim.lib
+    dt = np.float32(s.dt)
+    dv = _lib.gg_float3(*(float(dt * np.float32(g)) for g in s.gravity))
+    _lib.check("gg_add_external_force", lib.gg_add_external_force(C.byref(sim.set), dv, None))
+    out = torch.empty((s.n, 3), dtype=torch.float32, device=sim.dev)
+    _lib.check("gg_predict_positions",
+               lib.gg_predict_positions(C.byref(sim.set), float(dt), C.c_void_p(out.data_ptr()),
+                                        None))
+    _lib.check("gg_mass_scaling", lib.gg_mass_scaling(C.byref(sim.set), None))
+    torch.cuda.synchronize()
+    v1 = sim.vel[:s.n].cpu().numpy()
+    want_v = vel0 + np.array([dv.x, dv.y, dv.z], np.float32)
+    assert np.array_equal(v1, want_v)
+    assert np.array_equal(out.cpu().numpy(), pos0 + dt * v1)
+    assert np.array_equal(out.cpu().numpy(), xs)
+    ms = sim.scaled_mass[:s.n].cpu().numpy()
+    assert np.array_equal(ms, mass0 * np.exp(-pos0[:, 1].astype(np.float64)).astype(np.float32))
+    assert np.array_equal(ms, ms_fused)
+    # the positions alone sorted by given keys (out-of-range keys -> sentinel)
+    rng = np.random.default_rng(3)
+    C_ = sim.C
+    keys = rng.integers(-3, C_ + 4, size=s.n).astype(np.int32)
+    sim.p2c[:s.n] = torch.from_numpy(keys).to(sim.dev)
+    cs = torch.zeros(C_ + 2, dtype=torch.int32, device=sim.dev)
+    _lib.check("gg_sort_positions_by_key",
+               lib.gg_sort_positions_by_key(C.byref(sim.set), C.c_void_p(cs.data_ptr()),
+                                            C.byref(sim.ws), None))
+    torch.cuda.synchronize()
+    eff = np.where((keys < 0) | (keys > C_), C_, keys)
+    order = np.argsort(eff, kind="stable")
+    assert np.array_equal(sim.pos[:s.n].cpu().numpy(), pos0[order])
+    assert np.array_equal(sim.p2c[:s.n].cpu().numpy(), keys)
+    cnt = np.bincount(eff, minlength=C_ + 1)
+    assert np.array_equal(cs.cpu().numpy()[:C_ + 2], np.concatenate([[0], np.cumsum(cnt)]))
