@@ -9,6 +9,11 @@ the reference's constraint averaging cannot hold a 100-layer column: most
 particles are pushed through the floor lattice and leave the grid.)  A step = hash + stable sort + cell table + permute/predict +
 neighbour lists + 13 contact sweeps + finalize, inputs resident in HBM.
 
+Before any warm-up, the scene is advanced --settle steps (default 100)
+outside every timed region: the lattice starts 0.1 m above the floor and
+lands around step 71, so the timed window (steps settle+warmup ..
+settle+warmup+steps) measures the settled pile, not the free fall.
+
 Prints one JSON line.  `value` = particle-steps/s over the timed steps.
 Per-stage times come from hipEvents on the launch stream around every stage
 of every 10th timed step (--stage-every); the other steps carry no events.
@@ -330,6 +335,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--settle", type=int, default=100,
+                    help="steps run before the warm-up, outside every timing (the pile lands "
+                         "around step 71)")
+    ap.add_argument("--graph-steps", type=int, default=None,
+                    help="steps timed through the captured step graph after the timed region "
+                         "(default: --steps; 0 skips)")
     ap.add_argument("--side", type=int, default=100, help="lattice side (100 -> 1M particles)")
     ap.add_argument("--box", type=float, default=2.4)
     ap.add_argument("--dt", type=float, default=0.002)
@@ -406,9 +417,13 @@ def main():
     sim = PlainSim(scene, stab=S, solver=I, mu_s=mu_s, mu_k=mu_k)
     print(f"# scene {scene.name}: n={n} walls={nb} cells={cells}", file=sys.stderr, flush=True)
 
+    sim.step(args.settle)
+    torch.cuda.synchronize()
+    print(f"# settled: {args.settle} steps", file=sys.stderr, flush=True)
     sim.step(args.warmup)
     torch.cuda.synchronize()
     print(f"# warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    t_first = args.settle + args.warmup
 
     K = S + I
     # Per-stage hipEvents (on the launch stream) bracket every stage of every
@@ -469,6 +484,27 @@ def main():
     bps = bytes_per_particle_step(n, cells, nb, S, K)
     step_frac = value * bps / (HBM_PEAK_GBS * 1e9)
 
+    # the same fixed-N step captured into a hipGraph (gg_step_graph), timed
+    # right after the eager window by events on the launch stream
+    graph = None
+    gsteps = args.steps if args.graph_steps is None else args.graph_steps
+    if gsteps > 0:
+        sg = sim.graph(1)
+        sg.launch(2)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        sg.launch(gsteps)
+        e1.record()
+        torch.cuda.synchronize()
+        g_ms = e0.elapsed_time(e1) / gsteps
+        sg.close()
+        t_g = t_first + args.steps + 2
+        graph = {"steps": f"{t_g}..{t_g + gsteps}", "ms_per_step": round(g_ms, 4),
+                 "particle_steps_per_s": round(n / (g_ms * 1e-3), 1)}
+        print(f"# graph: {gsteps} steps at {g_ms:.4f} ms/step", file=sys.stderr, flush=True)
+
     pos, vel, mass = sim.host()
     assert np.isfinite(pos).all() and np.isfinite(vel).all(), "non-finite state"
 
@@ -485,19 +521,39 @@ def main():
         olib = po.load()
         p = po.params_from(scene, S, I, mu_s, mu_k)
         bpos, bcs = po.sorted_boundary(scene.boundary, scene.cell_length, scene.cells)
-        steps_done = 0
-        cp, cv, cm = pos, vel, mass
-        tc0 = time.perf_counter()
-        while steps_done < 8:
-            cp, cv, cm = po.plain_steps_f32(cp, cv, cm, bpos, bcs, p, 1)[:3]
-            steps_done += 1
-            if time.perf_counter() - tc0 >= args.cpu_baseline_seconds:
-                break
-        tc = time.perf_counter() - tc0
-        cpu = {"value": round(n * steps_done / tc, 1), "unit": "particle-steps/s",
-               "cores": olib.oracle_get_threads(), "kind": "port",
-               "sample": f"{steps_done} step(s) of the same 1M-particle state after the GPU "
-                         f"warmup+timed steps, FP32 oracle (C++/OpenMP, full 27-cell scans)"}
+
+        def cpu_rate(max_steps, budget_s):
+            """particle-steps/s of the FP32 oracle from the current state."""
+            done, cp, cv, cm = 0, pos, vel, mass
+            tc0 = time.perf_counter()
+            while done < max_steps:
+                cp, cv, cm = po.plain_steps_f32(cp, cv, cm, bpos, bcs, p, 1)[:3]
+                done += 1
+                if time.perf_counter() - tc0 >= budget_s:
+                    break
+            return n * done / (time.perf_counter() - tc0), done
+
+        olib.oracle_set_threads(0)
+        threads = olib.oracle_get_threads()
+        rate_all, steps_all = cpu_rate(8, args.cpu_baseline_seconds)
+        olib.oracle_set_threads(1)
+        rate_one, steps_one = cpu_rate(2, args.cpu_baseline_seconds)
+        olib.oracle_set_threads(0)
+        model = None
+        try:
+            for line in Path("/proc/cpuinfo").read_text().splitlines():
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = {"value": round(rate_all, 1), "unit": "particle-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{steps_all} step(s) on {threads} threads and {steps_one} on 1 thread "
+                         f"of the same 1M-particle state after the GPU runs, FP32 oracle "
+                         f"(C++/OpenMP, full 27-cell scans)",
+               "one_thread": round(rate_one, 1), "cpu_model": model,
+               "host_cpus": os.cpu_count()}
 
     out = {
         "metric": METRIC,
@@ -516,8 +572,9 @@ def main():
             "workload": f"plain box {args.side}^3 jittered lattice (r=0.01, seed 1) in a "
                         f"{args.box}^3 m box, {scene.cells[0]}^3 cells, {nb} wall particles, "
                         f"dt={args.dt}, g=-9.8, {S} stabilisation + {I} solver sweeps, "
-                        f"friction {mu_s}/{mu_k}, FP32, steps {args.warmup}.."
-                        f"{args.warmup + args.steps}, neighbour-list skin {sim.params.skin}",
+                        f"friction {mu_s}/{mu_k}, FP32, settled pile: steps {t_first}.."
+                        f"{t_first + args.steps} (after {args.settle} settling + "
+                        f"{args.warmup} warm-up steps), neighbour-list skin {sim.params.skin}",
             "particles": n, "boundary_particles": nb, "cells": cells,
             "parallelism": "single GPU",
         },
@@ -527,6 +584,7 @@ def main():
         "stage_sample": (f"{len(sampled)} of the {args.steps} timed steps (every {every}th)"
                          if stage_ms else None),
         "roofline": roofline,
+        "graph": graph,
         "cpu_baseline": cpu,
         "upsample": upsample,
         "batch": batch,

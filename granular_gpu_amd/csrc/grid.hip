@@ -422,20 +422,25 @@ extern "C" int gg_grid_build(gg_particles *set, gg_adaptive *ad, int *cell_start
   return grid_build_general(set, ad, cell_start, perm_out, ws, stream, ws->capacity);
 }
 
+namespace gg {
+int pack_boundary(const gg_particles *boundary, const gg_workspace *ws, hipStream_t st) {
+  if (boundary->n <= 0) return GG_OK;
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  gg_params dp;
+  gg_params_default(&dp);
+  k_pack_boundary<<<blocks_for(boundary->n), kBlock, 0, st>>>(
+      reinterpret_cast<const float *>(boundary->pos), boundary->n, dp.boundary_radius,
+      at<float4>(ws, L.bnd4));
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+}  // namespace gg
+
 extern "C" int gg_boundary_build(gg_particles *boundary, int *bcell_start,
                                  const gg_workspace *ws, hipStream_t stream) {
   if (!boundary || !bcell_start || !ws) return GG_ERR_INVALID;
   GG_TRY(grid_build_general(boundary, nullptr, bcell_start, nullptr, ws, stream,
                             ws->boundary_capacity));
-  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
-                         ws->bmax);
-  if (boundary->n > 0) {
-    gg_params dp;
-    gg_params_default(&dp);
-    k_pack_boundary<<<blocks_for(boundary->n), kBlock, 0, stream>>>(
-        reinterpret_cast<const float *>(boundary->pos), boundary->n, dp.boundary_radius,
-        at<float4>(ws, L.bnd4));
-    GG_LAUNCH_CHECK();
-  }
-  return GG_OK;
+  return pack_boundary(boundary, ws, stream);
 }

@@ -15,6 +15,9 @@ int gather_inplace(void *arr, int words, const int *perm, int n, void *tmp, hipS
 // sort_by_cell + in-place permutation of the set's payload arrays.
 int grid_build_general(gg_particles *set, gg_adaptive *ad, int *cell_start, int *perm_out,
                        const gg_workspace *ws, hipStream_t st, int capacity_limit);
+// The packed float4 copy of the (already sorted) boundary positions that
+// the contact kernels read, into the workspace.
+int pack_boundary(const gg_particles *boundary, const gg_workspace *ws, hipStream_t st);
 // Exclusive scan of m ints: out[0..m] (out[m] = total); in[] is cleared.
 int scan_exclusive(int *in, int m, int *out, const gg_workspace *ws, hipStream_t st);
 }  // namespace gg

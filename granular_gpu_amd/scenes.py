@@ -179,11 +179,12 @@ def batch_box_sizes(count: int = 64, nmin: int = 4096, nmax: int = 262144, seed:
     return out
 
 
-def batch_box(n_side: int, seed: int = 1) -> Scene:
+def batch_box(n_side: int, seed: int = 1, dt: float = 0.002, g: float = -9.8) -> Scene:
     """A plain box scene sized for n_side^3 particles (r = 0.01): box edge
-    max(0.024 n_side, 0.02 (n_side - 1) + 0.25), dt = 0.002, g = -9.8."""
+    max(0.024 n_side, 0.02 (n_side - 1) + 0.25), dt = 0.002 and g = -9.8
+    unless given."""
     box = max(0.024 * n_side, 0.02 * (n_side - 1) + 0.25)
-    return plain_box(n_side, float(np.float32(box)), seed=seed, dt=0.002, g=-9.8)
+    return plain_box(n_side, float(np.float32(box)), seed=seed, dt=dt, g=g)
 
 
 def _largest_remainder(weights, total: int) -> np.ndarray:
