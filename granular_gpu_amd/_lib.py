@@ -113,6 +113,8 @@ _PROTOS = {
     "gg_boundary_build": (C.c_int, [P(gg_particles), VP, P(gg_workspace), VP]),
     "gg_step": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, P(gg_particles), VP, P(gg_params),
                           P(gg_workspace), VP]),
+    "gg_step_noalloc": (C.c_int, [P(gg_particles), P(gg_adaptive), VP, P(gg_particles), VP,
+                                  gg_grid, P(gg_params), VP]),
     "gg_predict": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, P(gg_params),
                              P(gg_workspace), VP]),
     "gg_contact_sweep": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, P(gg_params), C.c_int,

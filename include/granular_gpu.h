@@ -182,6 +182,25 @@ int gg_step(gg_particles *set, gg_adaptive *ad, int *cell_start,
             const gg_params *params, const gg_workspace *ws,
             hipStream_t stream);
 
+/* No counterpart in the reference.  gg_step for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the workspace is a library-owned
+ * static device array of GG_NOALLOC_SCRATCH_BYTES per device, held until the
+ * process ends, instead of a caller buffer.  grid is the grid the cell
+ * tables are built on; boundary->pos must be in the cell order bcell_start
+ * describes (as gg_boundary_build leaves it).  Requests whose workspace
+ * (gg_workspace_bytes(n, boundary n, grid, stab + solver sweeps, 48, 16))
+ * exceeds the array return GG_ERR_CAPACITY before any device work, e.g.
+ * the 32,768-particle box with 14,408 walls needs 15.4 MiB.  Each call
+ * clears the counters it relies on and re-packs the boundary; the array
+ * keeps the last call's contents and is shared by all calls on its device,
+ * so calls that overlap on one device race and must be ordered by the
+ * caller.  Results equal gg_step's bit for bit. */
+#define GG_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int gg_step_noalloc(gg_particles *set, gg_adaptive *ad, int *cell_start,
+                    const gg_particles *boundary, const int *bcell_start, gg_grid grid,
+                    const gg_params *params, hipStream_t stream);
+
 /* ---- adaptive merge/split ----------------------------------------------- */
 /* Surface flags into set->surface (K5 find_cover_vector + K6 find_surface,
  * GranularSystem.cu:506-637).  Uses the cell table of the start of the step
