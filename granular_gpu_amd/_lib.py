@@ -168,6 +168,7 @@ _PROTOS = {
                                   C.c_size_t, P(C.c_int), VP]),
     "gg_mfma_probe": (C.c_int, [VP, VP, VP, VP]),
     "gg_bf16_round_check": (C.c_int, [C.c_uint64, C.c_uint64, VP, VP]),
+    "gg_fast_math_check": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, VP, VP]),
     "gg_compact_scratch_bytes": (C.c_size_t, [C.c_longlong]),
     "gg_compact": (C.c_int, [VP, VP, C.c_longlong, C.c_int, VP, VP, VP, C.c_size_t, VP]),
     "gg_fill_u32": (C.c_int, [VP, C.c_size_t, C.c_uint32, VP]),

@@ -129,8 +129,73 @@ struct SweepConst {
   int gmax, bmax, cap;
 };
 
+// Correctly rounded square root, reciprocal and quotient on restricted
+// domains.  The compiler's IEEE sequences (what sqrtf and '/' compile to
+// under -fhip-fp32-correctly-rounded-divide-sqrt) wrap a core in steps that
+// are identities on these domains: for sqrt, the input scaling below 2^-96
+// and the +-0 / +inf class check; for division, v_div_scale (no scaling when
+// numerator and denominator are normal, the numerator's biased exponent
+// exceeds 23, the exponent difference is below 96 and neither 1/d nor n/d
+// is denormal), v_div_fmas (a plain fma when not scaled) and v_div_fixup
+// (pass-through for finite normal quotients).  The cores below are those
+// sequences without the identity steps, so their results equal sqrtf and
+// '/' bit for bit on the domains (checked over every input by
+// gg_fast_math_check); callers fall back to sqrtf and '/' outside them.
+constexpr float kSqrtFastMin = 0x1p-96f;  // sqrt_core domain [2^-96, max float]
+
+__device__ __forceinline__ float sqrt_core(float x) {
+  float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __int_as_float(__float_as_int(s) - 1);
+  const float sp = __int_as_float(__float_as_int(s) + 1);
+  const float rm = __builtin_fmaf(-sm, s, x);
+  const float rp = __builtin_fmaf(-sp, s, x);
+  s = rm <= 0.0f ? sm : s;
+  s = rp > 0.0f ? sp : s;
+  return s;
+}
+
+// n / d for normal n, d with the conditions above.
+__device__ __forceinline__ float div_core(float n, float d) {
+  const float r0 = __builtin_amdgcn_rcpf(d);
+  const float e0 = __builtin_fmaf(-d, r0, 1.0f);
+  const float r1 = __builtin_fmaf(e0, r0, r0);
+  const float q0 = n * r1;
+  const float e1 = __builtin_fmaf(-d, q0, n);
+  const float q1 = __builtin_fmaf(e1, r1, q0);
+  const float e2 = __builtin_fmaf(-d, q1, n);
+  return __builtin_fmaf(e2, r1, q1);
+}
+
+// 1 / d (the numerator is exactly 1, so n * r1 = r1).
+__device__ __forceinline__ float rcp_core(float d) {
+  const float r0 = __builtin_amdgcn_rcpf(d);
+  const float e0 = __builtin_fmaf(-d, r0, 1.0f);
+  const float r1 = __builtin_fmaf(e0, r0, r0);
+  const float e1 = __builtin_fmaf(-d, r1, 1.0f);
+  const float q1 = __builtin_fmaf(e1, r1, r1);
+  const float e2 = __builtin_fmaf(-d, q1, 1.0f);
+  return __builtin_fmaf(e2, r1, q1);
+}
+
+// d = sqrt(d2) and inv = 1/d, as sqrtf and 1.0f/d give them.  For d2 in
+// [2^-96, max float], d lies in [2^-48, 2^64), where 1/d meets the
+// division conditions.
+__device__ __forceinline__ void sqrt_rcp(float d2, float &d, float &inv) {
+  if (d2 >= kSqrtFastMin && d2 <= 3.40282347e+38f) {
+    d = sqrt_core(d2);
+    inv = rcp_core(d);
+  } else {
+    d = sqrtf(d2);
+    inv = 1.0f / d;
+  }
+}
+
 // PBD friction (Macklin et al. 2014, sec. 6.1) on the relative displacement;
-// inv = 1/d of the contact.
+// inv = 1/d of the contact.  Static when |t|^2 < (mu_s delta)^2 (squares:
+// the static case needs no square root); otherwise t scaled by
+// min(mu_k delta / |t|, 1).  In the kinetic case |t| = sqrt(tl2) with tl2
+// in [2^-96, 2^80] and mu_k delta in [2^-80, 2^40] keep the quotient inside
+// the division domain (quotient in [2^-120, 2^88]).
 __device__ __forceinline__ float3 friction(float px, float py, float pz, float inv, float delta,
                                            float rx, float ry, float rz, float mu_s, float mu_k) {
   const float nx = px * inv, ny = py * inv, nz = pz * inv;
@@ -138,9 +203,15 @@ __device__ __forceinline__ float3 friction(float px, float py, float pz, float i
   float tx = rx - rn * nx;
   float ty = ry - rn * ny;
   float tz = rz - rn * nz;
-  const float tl = sqrtf(tx * tx + ty * ty + tz * tz);
-  if (tl < mu_s * delta) return make_float3(tx, ty, tz);
-  float c = (mu_k * delta) / tl;
+  const float tl2 = tx * tx + ty * ty + tz * tz;
+  const float lim = mu_s * delta;
+  if (tl2 < lim * lim) return make_float3(tx, ty, tz);
+  const float num = mu_k * delta;
+  float c;
+  if (tl2 >= kSqrtFastMin && tl2 <= 0x1p80f && num >= 0x1p-80f && num <= 0x1p40f)
+    c = div_core(num, sqrt_core(tl2));
+  else
+    c = num / sqrtf(tl2);
   c = fminf(c, 1.0f);
   return make_float3(tx * c, ty * c, tz * c);
 }
@@ -161,9 +232,9 @@ __device__ __forceinline__ bool boundary_terms(const float4 &bi, const float4 &o
   const float pz = (bi.z - q.z) + oi.z;
   const float d2 = px * px + py * py + pz * pz;
   if (!(d2 <= pre)) return false;
-  const float d = sqrtf(d2);
+  float d, inv;
+  sqrt_rcp(d2, d, inv);
   if (!(d < rc && d > 0.0f)) return false;
-  const float inv = 1.0f / d;
   const float s = (rc - d) * inv;
   c = make_float3(s * px, s * py, s * pz);
   if (FRIC) f = friction(px, py, pz, inv, rc - d, oi.x, oi.y, oi.z, sc.mu_s, sc.mu_k);
@@ -181,9 +252,9 @@ __device__ __forceinline__ bool granular_terms(const float4 &bi, const float4 &o
   const float d2 = px * px + py * py + pz * pz;
   const float rc = bi.w + bj.w;
   if (!(d2 <= contact_prefilter(rc))) return false;
-  const float d = sqrtf(d2);
+  float d, inv;
+  sqrt_rcp(d2, d, inv);
   if (!(d <= rc && d > 0.0f)) return false;
-  const float inv = 1.0f / d;
   const float s = (rc - d) * inv;
   c = make_float3(s * px, s * py, s * pz);
   if (FRIC) {

@@ -22,13 +22,14 @@ constexpr float kPiF = 3.14159265358979323846f;
 // the skin.
 constexpr float kListValidFrac = 0.9f;
 // A wave with at most this many particles that must scan the full stencil
-// queues them for one wave each (else each scans in its own lane): in a
-// sweep-wide queue drained by a second pass for sets of at least
-// kSweepQueueMinParticles, else in its block's LDS queue, scanned by the
-// block's waves after their list work (no second launch for small sets).
+// queues them for one wave each (else each scans in its own lane), in its
+// block's LDS queue, scanned by the block's waves after their list work.
+// GG_FLAG_FALLBACK_WAVE instead queues them sweep-wide for a second pass
+// after the sweep (at most kFallbackWaveMax per wave).  Measured on the
+// 1M-particle pile, the block queue is 6% faster at steps 110..115 and 2%
+// at 480..485 (no second launch per sweep), 3% slower at 300..305.
 constexpr int kFallbackWaveMax = 32;
 constexpr int kFallbackWaveMaxLocal = 6;
-constexpr int kSweepQueueMinParticles = 1 << 19;
 constexpr int kFallbackSlots = 62;  // ctr[1 + k % 62] = full-stencil count of sweep k
 // ctr[63..] are not used by the sweeps; the adaptive passes use ctr2 (own region)
 constexpr int kMergeRounds = 12;    // parallel claim-resolution rounds before the finisher

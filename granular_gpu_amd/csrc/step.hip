@@ -504,9 +504,10 @@ __global__ void __launch_bounds__(BS, 6) k_sweep(SweepArgs a) {
   }
   // Stencil scans of a few lanes are queued (one wave each): scanned in
   // their own lane they would hold this wave several times longer than its
-  // list work.  Large sets queue sweep-wide for k_sweep_queue, which spreads
-  // them over the grid; small sets queue in the block (LDS, at most BS
-  // entries), scanned by the block's waves at the end.  A sweep-wide slot
+  // list work.  They queue in the block (LDS, at most BS entries), scanned
+  // by the block's waves at the end, or with GG_FLAG_FALLBACK_WAVE
+  // sweep-wide for k_sweep_queue, which spreads them over the grid.  A
+  // sweep-wide slot
   // past the capacity (a queue that was not drained) scans in its lane.
   const unsigned long long scan = __ballot(mode == 2);
   bool queued = false;
@@ -805,7 +806,7 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   a.near_limit = 0.999f * ws->grid.cell_length;
   // GG_FLAG_FALLBACK_WAVE queues every stencil scan sweep-wide, whatever the
   // size of the set
-  a.fbq_global = (n >= kSweepQueueMinParticles || (p->flags & GG_FLAG_FALLBACK_WAVE)) ? 1 : 0;
+  a.fbq_global = (p->flags & GG_FLAG_FALLBACK_WAVE) ? 1 : 0;
   a.fb_wave_max = (p->flags & GG_FLAG_FALLBACK_LANE)   ? 0
                   : (p->flags & GG_FLAG_FALLBACK_WAVE) ? 64
                   : a.fbq_global                       ? kFallbackWaveMax

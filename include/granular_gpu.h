@@ -81,10 +81,10 @@ typedef struct gg_params {
 #define GG_FLAG_NO_XCD_REMAP 2 /* keep hardware block order (no XCD grouping) */
 /* Particles whose neighbour list is not valid for a sweep scan the 27-cell
  * stencil instead: one wave per particle when few lanes of a wave need it
- * (queued in the block for sets below 2^19 particles, else sweep-wide and
- * scanned by a second pass after the sweep), one lane per particle
- * otherwise.  These force one way (WAVE: every such particle queued
- * sweep-wide, whatever the size); results are the same bit for bit. */
+ * (queued in the block and scanned by its waves after their list work),
+ * one lane per particle otherwise.  LANE forces every such particle to scan
+ * in its lane; WAVE queues every one sweep-wide for a second pass after the
+ * sweep.  Results are the same bit for bit. */
 #define GG_FLAG_FALLBACK_LANE 4
 #define GG_FLAG_FALLBACK_WAVE 8
 
@@ -434,6 +434,16 @@ int gg_mfma_probe(const uint16_t *a, const uint16_t *b, float *d, hipStream_t st
  * round-to-nearest-even the header defines. */
 int gg_bf16_round_check(uint64_t first, uint64_t count, unsigned long long *mismatches,
                         hipStream_t stream);
+
+/* Adds to *mismatches (device counter) the number of inputs on which the
+ * contact kernels' short correctly rounded square root, reciprocal and
+ * quotient differ from the full IEEE sequences (sqrtf and '/'): mode 0,
+ * sqrt of every float bit pattern first .. first+count-1 (mod 2^32) in
+ * [2^-96, max float]; mode 1, 1/d of every pattern in [2^-48, 2^64);
+ * mode 2, n/d for count pseudo-random pairs derived from the patterns,
+ * n in [2^-80, 2^40], d in [2^-48, 2^40]. */
+int gg_fast_math_check(int mode, uint64_t first, uint64_t count, unsigned long long *mismatches,
+                       hipStream_t stream);
 
 /* ---- batched scenes -------------------------------------------------------- */
 /* Many independent plain scenes advanced together by one hipGraph launch

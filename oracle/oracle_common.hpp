@@ -155,7 +155,8 @@ template <class T> inline SweepParams<T> make_sweep_params(const oracle_params *
 
 // PBD friction (Macklin et al. 2014, "Unified Particle Physics for
 // Real-Time Applications", sec. 6.1): tangential part t of the relative
-// displacement rel; static if |t| < mu_s*delta, else t*min(mu_k*delta/|t|,1).
+// displacement rel; static if |t|^2 < (mu_s*delta)^2 (squares, so that the
+// static case needs no square root), else t*min(mu_k*delta/|t|,1).
 // inv = 1/d of the contact.
 template <class T>
 inline V3<T> friction(V3<T> p, T inv, T delta, V3<T> rel, const SweepParams<T> &sp) {
@@ -165,8 +166,10 @@ inline V3<T> friction(V3<T> p, T inv, T delta, V3<T> rel, const SweepParams<T> &
   t.x = rel.x - rn * nx;
   t.y = rel.y - rn * ny;
   t.z = rel.z - rn * nz;
-  const T tl = std::sqrt(t.x * t.x + t.y * t.y + t.z * t.z);
-  if (tl < sp.mu_s * delta) return t;
+  const T tl2 = t.x * t.x + t.y * t.y + t.z * t.z;
+  const T lim = sp.mu_s * delta;
+  if (tl2 < lim * lim) return t;
+  const T tl = std::sqrt(tl2);
   T c = (sp.mu_k * delta) / tl;
   c = std::fmin(c, (T)1);
   return V3<T>{t.x * c, t.y * c, t.z * c};

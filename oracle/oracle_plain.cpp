@@ -57,7 +57,8 @@ void grid_build(const V3<T> *pos, int n, const Grid &g, int *keys_sorted,
 //                                                       n  += 2
 // (own_ij + mirror_ji of the reference's scatter; mass scaling cancels).
 // Optional PBD friction in the solver phase removes the relative tangential
-// displacement (static if |t| < mu_s*delta, else scaled by mu_k*delta/|t|).
+// displacement (static if |t|^2 < (mu_s*delta)^2, else scaled by
+// mu_k*delta/|t|).
 // p = (b_i - b_j) + (o_i - o_j): positions are base + offset, never summed
 // before differencing, so relative geometry keeps full precision.
 //
@@ -88,12 +89,14 @@ inline V3<T> friction_m(V3<T> p, T inv, T delta, V3<T> rel, const SweepParams<T>
   t.x = rel.x - rn * nx;
   t.y = rel.y - rn * ny;
   t.z = rel.z - rn * nz;
-  const T tl = std::sqrt(t.x * t.x + t.y * t.y + t.z * t.z);
+  const T tl2 = t.x * t.x + t.y * t.y + t.z * t.z;
+  const T lim = sp.mu_s * delta;
   bool is_static;
   if (MODE == REPLAY) is_static = bits & 2;
-  else is_static = tl < sp.mu_s * delta;
+  else is_static = tl2 < lim * lim;
   if (MODE == RECORD && is_static) bits |= 2;
   if (is_static) return t;
+  const T tl = std::sqrt(tl2);
   T c = (sp.mu_k * delta) / tl;
   bool clamp;
   if (MODE == REPLAY) clamp = bits & 4;

@@ -312,3 +312,18 @@ def test_separate_stages_equal_fused_predict(gpu):
     assert np.array_equal(sim.p2c[:s.n].cpu().numpy(), keys)
     cnt = np.bincount(eff, minlength=C_ + 1)
     assert np.array_equal(cs.cpu().numpy()[:C_ + 2], np.concatenate([[0], np.cumsum(cnt)]))
+
+
+@pytest.mark.parametrize("mode,count", [(0, 1 << 32), (1, 1 << 32), (2, 1 << 32)])
+def test_short_sqrt_div_equal_ieee(gpu, mode, count):
+    # the contact kernels' short square root / reciprocal / quotient equal
+    # sqrtf and '/' on every input of their domains (mode 2: 2^32 pairs)
+    import torch
+    from granular_gpu_amd import _lib
+    from granular_gpu_amd.device import _ptr, _stream
+    _log(f"fast math check mode {mode}")
+    bad = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _lib.check("gg_fast_math_check",
+               _lib.load().gg_fast_math_check(mode, 0, count, _ptr(bad), _stream()))
+    torch.cuda.synchronize()
+    assert int(bad.item()) == 0
