@@ -51,11 +51,24 @@ void put(std::ofstream &f, const std::vector<T> &v) {
   if (!v.empty()) f.write(reinterpret_cast<const char *>(v.data()), sizeof(T) * v.size());
 }
 
+// Reads n elements after checking that the file still holds them (a
+// corrupt count must not size the vector beyond the file).
 template <class T>
-void get(std::ifstream &f, std::vector<T> &v, uint64_t n) {
+void get(std::ifstream &f, std::vector<T> &v, uint64_t n, uint64_t &left) {
+  if (n > left / sizeof(T)) throw std::runtime_error("checkpoint: truncated file");
   v.resize(n);
   if (n && !f.read(reinterpret_cast<char *>(v.data()), sizeof(T) * n))
     throw std::runtime_error("checkpoint: truncated file");
+  left -= sizeof(T) * n;
+}
+
+// A count field: at most INT32_MAX (counts cross the C-ABI as int).
+uint64_t get_count(std::ifstream &f, uint64_t &left) {
+  uint64_t n = 0;
+  if (left < 8 || !f.read(reinterpret_cast<char *>(&n), 8) || n > 0x7fffffffull)
+    throw std::runtime_error("checkpoint: bad count");
+  left -= 8;
+  return n;
 }
 
 }  // namespace
@@ -185,22 +198,21 @@ Checkpoint read_checkpoint(const std::string &path) {
   if (!f.read(reinterpret_cast<char *>(&version), 4) || version != kVersion)
     throw std::runtime_error("checkpoint: unsupported version");
   Checkpoint c;
-  uint64_t n = 0, nb = 0, nf = 0;
-  const uint64_t kMaxCount = 1ull << 34;
-  if (!f.read(reinterpret_cast<char *>(&n), 8) || n > kMaxCount)
-    throw std::runtime_error("checkpoint: bad count");
-  get(f, c.pos, n);
-  get(f, c.vel, n);
-  get(f, c.mass, n);
-  get(f, c.adaptive_last_step, n);
-  if (!f.read(reinterpret_cast<char *>(&nb), 8) || nb > kMaxCount)
-    throw std::runtime_error("checkpoint: bad count");
-  get(f, c.boundary_pos, nb);
-  get(f, c.boundary_animated, nb);
-  if (!f.read(reinterpret_cast<char *>(&nf), 8) || nf > kMaxCount)
-    throw std::runtime_error("checkpoint: bad count");
-  get(f, c.fine_pos, nf);
-  get(f, c.fine_vel, nf);
+  f.seekg(0, std::ios::end);
+  const uint64_t size = (uint64_t)f.tellg();
+  f.seekg(8, std::ios::beg);
+  uint64_t left = size - 8;
+  const uint64_t n = get_count(f, left);
+  get(f, c.pos, n, left);
+  get(f, c.vel, n, left);
+  get(f, c.mass, n, left);
+  get(f, c.adaptive_last_step, n, left);
+  const uint64_t nb = get_count(f, left);
+  get(f, c.boundary_pos, nb, left);
+  get(f, c.boundary_animated, nb, left);
+  const uint64_t nf = get_count(f, left);
+  get(f, c.fine_pos, nf, left);
+  get(f, c.fine_vel, nf, left);
   char extra;
   if (f.read(&extra, 1)) throw std::runtime_error("checkpoint: trailing bytes");
   return c;

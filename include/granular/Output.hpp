@@ -62,7 +62,12 @@ struct Checkpoint {
 };
 
 // Writes / reads the checkpoint file.  Reading checks the magic, version,
-// counts and length; throws std::runtime_error on any mismatch.
+// counts (at most INT32_MAX, and never more than the bytes left in the
+// file) and length; throws std::runtime_error on any mismatch.  The file
+// holds the state a plain step starts from; the merge/split bookkeeping of
+// an adaptive run (remove flags, merge masses and velocities, blend counts)
+// is not in it, so an adaptive run resumes equivalently only from a step
+// where no merge blend is in flight.
 void write_checkpoint(const std::string &path, const Checkpoint &c);
 Checkpoint read_checkpoint(const std::string &path);
 

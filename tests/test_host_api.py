@@ -263,6 +263,14 @@ def test_checkpoint_round_trip(tmp_path):
     (tmp_path / "short.ggck").write_bytes(raw[:-1])
     (tmp_path / "magic.ggck").write_bytes(b"XXCK" + raw[4:])
     (tmp_path / "long.ggck").write_bytes(raw + b"\0")
-    for bad in ("short", "magic", "long"):
+    # a count far beyond the file (would size a vector of 2^33 float3) and
+    # one above INT32_MAX are refused before anything is allocated
+    huge = bytearray(raw)
+    huge[8:16] = (1 << 33).to_bytes(8, "little")
+    (tmp_path / "huge.ggck").write_bytes(bytes(huge))
+    wide = bytearray(raw)
+    wide[8:16] = (1 << 31).to_bytes(8, "little")
+    (tmp_path / "wide.ggck").write_bytes(bytes(wide))
+    for bad in ("short", "magic", "long", "huge", "wide"):
         with pytest.raises(host.HostError):
             host.read_checkpoint(tmp_path / f"{bad}.ggck")
