@@ -43,7 +43,9 @@ class gg_params(C.Structure):
         ("contact_radius_min", C.c_float), ("sleep_speed", C.c_float),
         ("damp_speed", C.c_float), ("damping", C.c_float), ("adaptive_damping", C.c_float),
         ("skin", C.c_float), ("flags", C.c_int), ("sweep_block", C.c_int),
-        ("nlist_block", C.c_int),
+        ("nlist_block", C.c_int), ("adaptive_rules", C.c_int), ("min_mass", C.c_float),
+        ("split_mass", C.c_float), ("merge_dist", C.c_float), ("merge_depth", C.c_float),
+        ("split_depth", C.c_float),
     ]
 
 

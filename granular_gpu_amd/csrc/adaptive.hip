@@ -121,6 +121,39 @@ __global__ void __launch_bounds__(kBlock)
   surf[i] = (nn <= 5 || !vote) ? 1 : 0;
 }
 
+// Merge / split rules.  Reference rules (Solver.cu:526-653, 780-827):
+// "deep" = not surface and no surface particle in the stencil (sd > 100:
+// sd is a squared distance or the sentinel 1e5), partners within 0.035 m,
+// merging needs 1 <= m < max_mass, splitting a surface particle with m > 2.
+// Depth rules (radius levels): r = the surface radius of the mass; deep =
+// not surface and sd > (merge_depth r)^2, partners within merge_dist
+// (r_i + r_j), splitting needs sd < (split_depth r)^2 and m > split_mass.
+struct MergeRules {
+  int depth;
+  float min_mass, split_mass, merge_dist, merge_depth, split_depth;
+};
+
+inline MergeRules rules_of(const gg_params *p) {
+  MergeRules R;
+  R.depth = p->adaptive_rules;
+  R.min_mass = p->min_mass;
+  R.split_mass = p->split_mass;
+  R.merge_dist = p->merge_dist;
+  R.merge_depth = p->merge_depth;
+  R.split_depth = p->split_depth;
+  return R;
+}
+
+__device__ __forceinline__ bool deep(const MergeRules &R, int surf, float sd, float r) {
+  if (!R.depth) return surf == 0 && sd > 100.0f;
+  const float t = R.merge_depth * r;
+  return surf == 0 && sd > t * t;
+}
+
+__device__ __forceinline__ float partner_dist(const MergeRules &R, float ri, float rj) {
+  return R.depth ? R.merge_dist * (ri + rj) : R.merge_dist;
+}
+
 // K7: squared distance to the nearest surface particle of the stencil.
 __global__ void __launch_bounds__(kBlock)
     k_surface_distance(int n, const float *__restrict__ pos, const int *__restrict__ surf,
@@ -147,9 +180,9 @@ __global__ void __launch_bounds__(kBlock)
     k_merge_cand(int n, const float *__restrict__ pos, const float *__restrict__ mass,
                  const int *__restrict__ surf, const float *__restrict__ sd,
                  const int *__restrict__ remove, const int *__restrict__ cs, DevGrid g,
-                 float max_mass, int *__restrict__ cand, float *__restrict__ dens,
-                 int *__restrict__ state, int *__restrict__ owner, int *__restrict__ und0,
-                 int *__restrict__ und1) {
+                 float max_mass, MergeRules R, const float *__restrict__ rs,
+                 int *__restrict__ cand, float *__restrict__ dens, int *__restrict__ state,
+                 int *__restrict__ owner, int *__restrict__ und0, int *__restrict__ und1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   owner[i] = INT_MAX;
@@ -158,9 +191,11 @@ __global__ void __launch_bounds__(kBlock)
   cand[i] = -1;
   state[i] = M_NONE;
   const float mi = mass[i];
-  if (surf[i] != 0 || remove[i] != 0 || mi < 1.0f || mi >= max_mass || sd[i] < 100.0f) return;
+  const float ri = R.depth ? rs[i] : 0.0f;
+  if (surf[i] != 0 || remove[i] != 0 || mi < R.min_mass || mi >= max_mass ||
+      !deep(R, surf[i], sd[i], ri))
+    return;
   const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
-  const float sdi = sd[i];
   float closest = 1000.0f;
   int ci = -1;
   float ld = 0.0f;
@@ -169,13 +204,14 @@ __global__ void __launch_bounds__(kBlock)
     if (j >= n) return;
     const float dx = px - pos[3 * j], dy = py - pos[3 * j + 1], dz = pz - pos[3 * j + 2];
     const float dis = sqrtf(dx * dx + dy * dy + dz * dz);
-    if (dis < 0.035f) {
+    const float lim = partner_dist(R, ri, R.depth ? rs[j] : 0.0f);
+    if (dis < lim) {
       ld = ld + mass[j];
       ++lc;
     }
-    if (j <= i || remove[j] != 0 || surf[j] != 0 || sdi < 100.0f) return;
+    if (j <= i || remove[j] != 0 || surf[j] != 0) return;
     if (mi + mass[j] > max_mass) return;
-    if (dis < 0.035f && dis < closest) {
+    if (dis < lim && dis < closest) {
       closest = dis;
       ci = j;
     }
@@ -207,7 +243,7 @@ __global__ void __launch_bounds__(kBlock)
     k_merge_decide(int n, const int *__restrict__ cand, int *__restrict__ state,
                    const int *__restrict__ owner, const int *__restrict__ und,
                    int *__restrict__ und_next, const int *__restrict__ surf,
-                   const float *__restrict__ sd) {
+                   const float *__restrict__ sd, MergeRules R, const float *__restrict__ rs) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   und_next[i] = INT_MAX;
@@ -217,7 +253,7 @@ __global__ void __launch_bounds__(kBlock)
     return;
   }
   if (und[i] != INT_MAX) return;
-  if (!(surf[i] == 0 && sd[i] > 100.0f)) {
+  if (!deep(R, surf[i], sd[i], R.depth ? rs[i] : 0.0f)) {
     state[i] = M_SELF_LOCK;
     return;
   }
@@ -227,7 +263,7 @@ __global__ void __launch_bounds__(kBlock)
     return;
   }
   if (und[c] < i) return;
-  state[i] = (surf[c] == 0 && sd[c] > 100.0f) ? M_SUCCESS : M_PARTNER_LOCK;
+  state[i] = deep(R, surf[c], sd[c], R.depth ? rs[c] : 0.0f) ? M_SUCCESS : M_PARTNER_LOCK;
 }
 
 // Ordered list of the still-undecided candidates (one block per 256
@@ -266,7 +302,8 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void k_merge_finish(const int *__restrict__ list, const int *__restrict__ count,
                                const int *__restrict__ cand, int *__restrict__ state,
                                int *__restrict__ owner, const int *__restrict__ surf,
-                               const float *__restrict__ sd) {
+                               const float *__restrict__ sd, MergeRules R,
+                               const float *__restrict__ rs) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const int m = *count;
   for (int k = 0; k < m; ++k) {
@@ -275,12 +312,12 @@ __global__ void k_merge_finish(const int *__restrict__ list, const int *__restri
     int st;
     if (owner[i] != INT_MAX)
       st = M_CLAIMED;
-    else if (!(surf[i] == 0 && sd[i] > 100.0f))
+    else if (!deep(R, surf[i], sd[i], R.depth ? rs[i] : 0.0f))
       st = M_SELF_LOCK;
     else if (owner[c] < i)
       st = M_RELEASE;
     else
-      st = (surf[c] == 0 && sd[c] > 100.0f) ? M_SUCCESS : M_PARTNER_LOCK;
+      st = deep(R, surf[c], sd[c], R.depth ? rs[c] : 0.0f) ? M_SUCCESS : M_PARTNER_LOCK;
     state[i] = st;
     if ((st == M_SUCCESS || st == M_PARTNER_LOCK) && i < owner[c]) owner[c] = i;
   }
@@ -357,6 +394,7 @@ __global__ void __launch_bounds__(kBlock)
 __global__ void __launch_bounds__(kBlock)
     k_split_check(int n, const float *__restrict__ pos, const float *__restrict__ mass,
                   const int *__restrict__ remove, const int *__restrict__ surf,
+                  const float *__restrict__ sd, const float *__restrict__ rs, MergeRules R,
                   const float4 *__restrict__ bnd4, const int *__restrict__ cs,
                   const int *__restrict__ bcs, DevGrid g, float denom_f,
                   int *__restrict__ ok, float4 *__restrict__ child, int *__restrict__ bsurv,
@@ -367,7 +405,14 @@ __global__ void __launch_bounds__(kBlock)
   if (i < n) {
     surv = remove[i] != 1;
     const float m = mass[i];
-    if (remove[i] == 0 && surf[i] == 1 && m > 2.0f) {
+    bool want = remove[i] == 0 && m > R.split_mass;
+    if (want && R.depth) {
+      const float t = R.split_depth * rs[i];
+      want = sd[i] < t * t;
+    } else if (want) {
+      want = surf[i] == 1;
+    }
+    if (want) {
       const float r = cbrt_f((3.0f * m) / denom_f);
       const float maxd = 2.0f * r;
       const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
@@ -582,8 +627,21 @@ static int scan_blocks(int *bsum, int nblocks, int *out, const gg_workspace *ws,
   return scan_exclusive(bsum, nblocks, out, ws, st);
 }
 
-extern "C" int gg_merge_mark(gg_particles *set, gg_adaptive *ad, const int *cell_start,
-                             float max_mass, const gg_workspace *ws, hipStream_t stream) {
+// Surface radii of the set (the depth rules' r) into the workspace.
+static int merge_radii(const gg_particles *set, float density, const gg_workspace *ws,
+                       hipStream_t stream, const float **out) {
+  Layout L = layout_of(ws);
+  float *rs = at<float>(ws, L.mrad);
+  const double denom = 4.0 * kPiD * (double)density;
+  k_radius_surface<<<blocks_for(set->n), kBlock, 0, stream>>>(set->n, set->mass, denom, rs);
+  GG_LAUNCH_CHECK();
+  *out = rs;
+  return GG_OK;
+}
+
+static int merge_mark_rules(gg_particles *set, gg_adaptive *ad, const int *cell_start,
+                            float max_mass, const MergeRules &R, const float *rs,
+                            const gg_workspace *ws, hipStream_t stream) {
   GG_TRY(check_adaptive_set(set, ws));
   if (!ad || !ad->remove || !ad->merge || !ad->merge_velocity || !cell_start || !set->vel ||
       !set->surface || !set->surface_distance)
@@ -599,8 +657,8 @@ extern "C" int gg_merge_mark(gg_particles *set, gg_adaptive *ad, const int *cell
   const float *pos = reinterpret_cast<const float *>(set->pos);
   prof_begin(GG_STAGE_ADAPTIVE, stream);
   k_merge_cand<<<nb, kBlock, 0, stream>>>(n, pos, set->mass, set->surface, set->surface_distance,
-                                          ad->remove, cell_start, g, max_mass, cand, dens, state,
-                                          owner, und0, und1);
+                                          ad->remove, cell_start, g, max_mass, R, rs, cand, dens,
+                                          state, owner, und0, und1);
   GG_LAUNCH_CHECK();
   for (int r = 0; r < kMergeRounds; ++r) {
     int *u = (r % 2 == 0) ? und0 : und1;
@@ -608,7 +666,7 @@ extern "C" int gg_merge_mark(gg_particles *set, gg_adaptive *ad, const int *cell
     k_merge_fold<<<nb, kBlock, 0, stream>>>(n, cand, state, owner, u);
     GG_LAUNCH_CHECK();
     k_merge_decide<<<nb, kBlock, 0, stream>>>(n, cand, state, owner, u, un, set->surface,
-                                              set->surface_distance);
+                                              set->surface_distance, R, rs);
     GG_LAUNCH_CHECK();
   }
   // fold the last round's decisions, then finish the rest in index order
@@ -623,7 +681,7 @@ extern "C" int gg_merge_mark(gg_particles *set, gg_adaptive *ad, const int *cell
   k_list_und<<<nb, kBlock, 0, stream>>>(n, state, boff, list);
   GG_LAUNCH_CHECK();
   k_merge_finish<<<1, 64, 0, stream>>>(list, boff + nb, cand, state, owner, set->surface,
-                                       set->surface_distance);
+                                       set->surface_distance, R, rs);
   GG_LAUNCH_CHECK();
   k_merge_apply<<<nb, kBlock, 0, stream>>>(n, cand, state, dens, set->mass,
                                            reinterpret_cast<const float *>(set->vel), ad->remove,
@@ -632,6 +690,13 @@ extern "C" int gg_merge_mark(gg_particles *set, gg_adaptive *ad, const int *cell
   GG_LAUNCH_CHECK();
   prof_end(GG_STAGE_ADAPTIVE, stream);
   return GG_OK;
+}
+
+extern "C" int gg_merge_mark(gg_particles *set, gg_adaptive *ad, const int *cell_start,
+                             float max_mass, const gg_workspace *ws, hipStream_t stream) {
+  gg_params ref;
+  gg_params_default(&ref);
+  return merge_mark_rules(set, ad, cell_start, max_mass, rules_of(&ref), nullptr, ws, stream);
 }
 
 extern "C" int gg_merge_blend(gg_particles *set, gg_adaptive *ad, int blend_steps,
@@ -651,14 +716,16 @@ extern "C" int gg_merge_blend(gg_particles *set, gg_adaptive *ad, int blend_step
   return GG_OK;
 }
 
-extern "C" int gg_split_compact(gg_particles *set, gg_adaptive *ad, const int *cell_start,
-                                const gg_particles *boundary, const int *bcell_start,
-                                const gg_params *params, const gg_workspace *ws,
-                                hipStream_t stream, int *stats) {
+static int split_compact_rules(gg_particles *set, gg_adaptive *ad, const int *cell_start,
+                               const gg_particles *boundary, const int *bcell_start,
+                               const gg_params *params, const float *rs,
+                               const gg_workspace *ws, hipStream_t stream, int *stats) {
   GG_TRY(check_adaptive_set(set, ws));
   if (!ad || !ad->remove || !ad->merge || !ad->merge_count || !cell_start || !boundary ||
       !bcell_start || !params || !set->vel || !set->surface || !set->adaptive_last_step)
     return GG_ERR_INVALID;
+  const MergeRules R = rules_of(params);
+  if (R.depth && (!set->surface_distance || !rs)) return GG_ERR_INVALID;
   const int n = set->n;
   const int cap = set->capacity < ws->capacity ? set->capacity : ws->capacity;
   Layout L = layout_of(ws);
@@ -674,8 +741,8 @@ extern "C" int gg_split_compact(gg_particles *set, gg_adaptive *ad, const int *c
     prof_begin(GG_STAGE_ADAPTIVE, stream);
     k_split_check<<<nb, kBlock, 0, stream>>>(
         n, reinterpret_cast<const float *>(set->pos), set->mass, ad->remove, set->surface,
-        at<float4>(ws, L.bnd4), cell_start, bcell_start, g, denom_f, ok, at<float4>(ws, L.child),
-        bsurv, bsplit);
+        set->surface_distance, rs, R, at<float4>(ws, L.bnd4), cell_start, bcell_start, g, denom_f,
+        ok, at<float4>(ws, L.child), bsurv, bsplit);
     GG_LAUNCH_CHECK();
     GG_TRY(scan_exclusive(bsurv, nb, osurv, ws, stream));
     GG_TRY(scan_exclusive(bsplit, nb, osplit, ws, stream));
@@ -707,12 +774,34 @@ extern "C" int gg_split_compact(gg_particles *set, gg_adaptive *ad, const int *c
   return GG_OK;
 }
 
+extern "C" int gg_split_compact(gg_particles *set, gg_adaptive *ad, const int *cell_start,
+                                const gg_particles *boundary, const int *bcell_start,
+                                const gg_params *params, const gg_workspace *ws,
+                                hipStream_t stream, int *stats) {
+  if (!params || !set) return GG_ERR_INVALID;
+  const float *rs = nullptr;
+  if (params->adaptive_rules && set->n > 0) {
+    GG_TRY(check_adaptive_set(set, ws));
+    GG_TRY(merge_radii(set, params->density, ws, stream, &rs));
+  }
+  return split_compact_rules(set, ad, cell_start, boundary, bcell_start, params, rs, ws, stream,
+                             stats);
+}
+
 extern "C" int gg_adaptive_sampling(gg_particles *set, gg_adaptive *ad, const int *cell_start,
                                     const gg_particles *boundary, const int *bcell_start,
                                     const gg_params *params, float max_mass, int do_merge,
                                     int blend_steps, const gg_workspace *ws, hipStream_t stream,
                                     int *stats) {
-  if (do_merge) GG_TRY(gg_merge_mark(set, ad, cell_start, max_mass, ws, stream));
+  if (!params || !set) return GG_ERR_INVALID;
+  const MergeRules R = rules_of(params);
+  const float *rs = nullptr;
+  if (R.depth && set->n > 0) {
+    GG_TRY(check_adaptive_set(set, ws));
+    GG_TRY(merge_radii(set, params->density, ws, stream, &rs));
+  }
+  if (do_merge) GG_TRY(merge_mark_rules(set, ad, cell_start, max_mass, R, rs, ws, stream));
   GG_TRY(gg_merge_blend(set, ad, blend_steps, ws, stream));
-  return gg_split_compact(set, ad, cell_start, boundary, bcell_start, params, ws, stream, stats);
+  return split_compact_rules(set, ad, cell_start, boundary, bcell_start, params, rs, ws, stream,
+                             stats);
 }

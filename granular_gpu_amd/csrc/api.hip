@@ -60,6 +60,12 @@ extern "C" void gg_params_default(gg_params *p) {
   p->adaptive_damping = 0.98f;    // Solver.cu:242
   p->skin = 0.0075f;
   p->flags = 0;
+  p->adaptive_rules = 0;  // reference merge / split rules
+  p->min_mass = 1.0f;     // Solver.cu:542
+  p->split_mass = 2.0f;   // Solver.cu:795
+  p->merge_dist = 0.035f; // Solver.cu:572
+  p->merge_depth = 3.0f;
+  p->split_depth = 2.0f;
 }
 
 extern "C" size_t gg_workspace_bytes(int capacity, int boundary_capacity, gg_grid grid,

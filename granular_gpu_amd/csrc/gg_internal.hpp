@@ -78,6 +78,7 @@ struct Layout {
   size_t bsum;     // int[blocks+1]  per-block counts for ordered compaction
   size_t bsum2;    // int[blocks+1]
   size_t child;    // float4[cap]    split child positions
+  size_t mrad;     // float[cap]     surface radii for the depth merge / split rules
   size_t itmp2;    // int[cmax]
   size_t itmp3;    // int[cmax]
   size_t total;
@@ -144,6 +145,7 @@ inline Layout make_layout(int capacity, int bcap, gg_grid g, int max_sweeps,
   L.bsum = take(nblk * 4);
   L.bsum2 = take(nblk * 4);
   L.child = take(cap * 16);
+  L.mrad = take(cap * 4);
   L.itmp2 = take(cmax * 4);
   L.itmp3 = take(cmax * 4);
   L.total = o;

@@ -234,12 +234,15 @@ def cone_funnel(count: int, half_angle_deg: float = 30.0, outlet: float = 0.1,
 
 def funnel_pour(n: int = 8000, cone_count: int = 12000, r: float = 0.01, box: float = 2.0,
                 seed: int = 2, dt: float = 0.002, g: float = -9.8,
-                half_angle_deg: float = 30.0, outlet: float = 0.1) -> Scene:
+                half_angle_deg: float = 30.0, outlet: float = 0.1,
+                cell_length: float | None = None) -> Scene:
     """Funnel pour: a cone_funnel of `cone_count` particles (axis at the
     box centre in x and z, outlet at 0.3 box, height 0.3 box) inside BOX
     walls, and a column of n particles at seeded uniform-random positions
     in a cylinder above it: radius 0.8 of the cone's rim radius, from 0.02
-    above the rim, tall enough for (2.4 r)^3 of volume per particle."""
+    above the rim, tall enough for (2.4 r)^3 of volume per particle.
+    cell_length defaults to the scenes' 0.0404."""
+    L = CELL_LENGTH if cell_length is None else np.float32(cell_length)
     space = (box, box, box)
     h = 0.3 * box
     centre = (0.5 * box, 0.3 * box, 0.5 * box)
@@ -255,12 +258,12 @@ def funnel_pour(n: int = 8000, cone_count: int = 12000, r: float = 0.01, box: fl
     phi = 2.0 * math.pi * u[:, 1]
     pos = np.stack([centre[0] + rad * np.cos(phi), y0 + tall * u[:, 2],
                     centre[2] + rad * np.sin(phi)], 1).astype(np.float32)
-    walls = box_walls(space, CELL_LENGTH, "box")
+    walls = box_walls(space, L, "box")
     return Scene(
         name=f"funnel_pour_{n}_cone_{cone_count}", pos=pos, vel=np.zeros((n, 3), np.float32),
         mass=np.full(n, radius_mass(r), np.float32),
         boundary=np.concatenate([walls, cone]).astype(np.float32), space=space,
-        cell_length=CELL_LENGTH, cells=cell_counts(space), dt=np.float32(dt),
+        cell_length=L, cells=cell_counts(space, L), dt=np.float32(dt),
         gravity=(0.0, float(np.float32(g)), 0.0), density=DENSITY_R001, radius=r)
 
 

@@ -75,6 +75,21 @@ typedef struct gg_params {
                                64, 128, 256 or 512; 0 = 64 (autotuned by
                                gg_batch_autotune)                           */
   int nlist_block;          /* same for the neighbour-list build; 0 = 64    */
+  /* Merge / split rules of gg_adaptive_sampling and gg_split_compact
+   * (gg_merge_mark applies the reference rules).  Reference rules
+   * (Solver.cu:526-653, 780-827): merge partners deep (not surface, no
+   * surface particle in the stencil) within merge_dist = 0.035 m, merging
+   * needs min_mass <= m < max_mass, splitting a surface particle with
+   * m > split_mass = 2.  Depth rules (radius levels, r the radius of the
+   * mass): merge when sd > (merge_depth r)^2 with partners within
+   * merge_dist (r_i + r_j), split when sd < (split_depth r)^2 and
+   * m > split_mass; sd is the squared distance to the surface. */
+  int adaptive_rules;       /* 0 reference rules, 1 depth rules             */
+  float min_mass;           /* 1.0                                          */
+  float split_mass;         /* 2.0                                          */
+  float merge_dist;         /* 0.035 m (depth rules: a factor, e.g. 1.75)   */
+  float merge_depth;        /* 3.0 (depth rules)                            */
+  float split_depth;        /* 2.0 (depth rules)                            */
 } gg_params;
 
 #define GG_FLAG_NO_NLIST 1     /* always scan the 27-cell stencil */

@@ -44,6 +44,14 @@ typedef struct oracle_params {
   float adaptive_damping;   /* 0.98 (Solver.cu:242)                          */
   float skin;               /* neighbour-list skin (only for oracle_nlist)   */
   int own_only;             /* 0: symmetric gather; 1: own term only        */
+  /* merge / split rules (oracle_merge_mark, oracle_split_compact) */
+  int depth_rules;          /* 0: reference rules; 1: depth rules             */
+  float min_mass;           /* merging needs m >= min_mass (reference 1)      */
+  float split_mass;         /* splitting needs m > split_mass (reference 2)   */
+  float merge_dist;         /* partners within merge_dist (reference 0.035 m);
+                               depth rules: within merge_dist (r_i + r_j)     */
+  float merge_depth;        /* depth rules: merge when sd > (merge_depth r)^2 */
+  float split_depth;        /* depth rules: split when sd < (split_depth r)^2 */
 } oracle_params;
 
 /* Grid build: hash, stable sort by cell id, cell_start (C+2 entries).
@@ -120,10 +128,12 @@ int oracle_merge_mark(int n, const float *pos, const float *mass,
 int oracle_merge_blend(int n, float *merge, float *mass, float *vel,
                        int *merge_count, int *remove,
                        const float *merge_velocity, int blend, int *adl);
-/* K10/K11 + compaction + append; returns the new particle count.          */
+/* K10/K11 + compaction + append; returns the new particle count.  sd (the
+ * squared surface distances) is read by the depth rules only (may be NULL
+ * with the reference rules).                                              */
 int oracle_split_compact(int n, int cap, float *pos, float *vel, float *mass,
                          int *adl, int *remove, float *merge,
-                         int *merge_count, const int *surface,
+                         int *merge_count, const int *surface, const float *sd,
                          const float *bpos, const int *cs, const int *bcs,
                          const oracle_params *p, int *stats);
 /* Whole adaptive steps (plain step + the passes above).  stats[3k..3k+2] =

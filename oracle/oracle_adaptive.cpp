@@ -127,15 +127,21 @@ void surface_distance(int n, const V3<float> *pos, const int *surf, const int *c
 // K8 (Solver.cu:526-653) with claims resolved in ascending i.
 void merge_mark(int n, const V3<float> *pos, const float *mass, const V3<float> *vel,
                 const int *surf, const float *sd, int *remove, float *merge, V3<float> *mv,
-                const int *cs, const Grid &g, float max_mass, int *cand_out,
+                const int *cs, const Grid &g, float max_mass, const Rules &R, int *cand_out,
                 float *dens_out) {
   std::vector<int> cand(n, -1);
   std::vector<float> dens(n, 0.0f);
+  std::vector<float> rs(R.depth ? n : 0);
+  if (R.depth)
+    for (int i = 0; i < n; ++i) rs[i] = radius_surface_f(mass[i], R.density);
+  auto rad = [&](int i) { return R.depth ? rs[i] : 0.0f; };
   // candidates from the pre-kernel snapshot of remove[]
 #pragma omp parallel for schedule(dynamic, 256)
   for (int i = 0; i < n; ++i) {
     const float mi = mass[i];
-    if (surf[i] != 0 || remove[i] != 0 || mi < 1.0f || mi >= max_mass || sd[i] < 100.0f) continue;
+    if (surf[i] != 0 || remove[i] != 0 || mi < R.min_mass || mi >= max_mass ||
+        !deep(R, surf[i], sd[i], rad(i)))
+      continue;
     float closest = 1000.0f;
     int ci = -1;
     float ld = 0.0f;
@@ -143,13 +149,14 @@ void merge_mark(int n, const V3<float> *pos, const float *mass, const V3<float> 
     for_stencil(pos[i], g, cs, [&](int j) {
       if (j >= n) return;
       const float dis = len3(sub3(pos[i], pos[j]));
-      if (dis < 0.035f) {
+      const float lim = partner_dist(R, rad(i), rad(j));
+      if (dis < lim) {
         ld = ld + mass[j];
         ++lc;
       }
-      if (j <= i || remove[j] != 0 || surf[j] != 0 || sd[i] < 100.0f) return;
+      if (j <= i || remove[j] != 0 || surf[j] != 0) return;
       if (mi + mass[j] > max_mass) return;
-      if (dis < 0.035f && dis < closest) {
+      if (dis < lim && dis < closest) {
         closest = dis;
         ci = j;
       }
@@ -166,13 +173,13 @@ void merge_mark(int n, const V3<float> *pos, const float *mass, const V3<float> 
     if (c < 0) continue;
     if (rem[i] != 0) continue;  // claimed as a target earlier
     rem[i] = -1;
-    if (!(surf[i] == 0 && sd[i] > 100.0f)) continue;  // stays locked, no partner
+    if (!deep(R, surf[i], sd[i], rad(i))) continue;  // stays locked, no partner
     if (rem[c] != 0) {
       rem[i] = 0;
       continue;
     }
     rem[c] = -1;
-    if (surf[c] == 0 && sd[c] > 100.0f) {
+    if (deep(R, surf[c], sd[c], rad(c))) {
       const float mi = mass[i], mc = mass[c];
       merge[c] = mi;
       merge[i] = -mi;
@@ -225,13 +232,20 @@ void merge_blend(int n, float *merge, float *mass, V3<float> *vel, int *mcount, 
 }
 
 // K10 check (Solver.cu:708-827): surface particles with remove == 0 and
-// m > 2; child at the centre of the first empty (granular and boundary)
+// m > 2 (depth rules: within split_depth r of the surface and m >
+// split_mass); child at the centre of the first empty (granular and boundary)
 // non-centre stencil cell; no boundary particle within 2 r.
 bool split_check(int i, const V3<float> *pos, const float *mass, const int *remove,
-                 const int *surf, const V3<float> *bpos, const int *cs, const int *bcs,
-                 const Grid &g, float density, V3<float> &child) {
-  if (remove[i] != 0 || surf[i] != 1 || !(mass[i] > 2.0f)) return false;
-  const float r = radius_split_f(mass[i], density);
+                 const int *surf, const float *sd, const V3<float> *bpos, const int *cs,
+                 const int *bcs, const Grid &g, const Rules &R, V3<float> &child) {
+  if (remove[i] != 0 || !(mass[i] > R.split_mass)) return false;
+  if (R.depth) {
+    const float t = R.split_depth * radius_surface_f(mass[i], R.density);
+    if (!(sd[i] < t * t)) return false;
+  } else if (surf[i] != 1) {
+    return false;
+  }
+  const float r = radius_split_f(mass[i], R.density);
   const float maxd = 2.0f * r;
   int bx, by, bz;
   if (!trunc_cell<float>(pos[i], g.L, bx, by, bz)) return false;
@@ -276,7 +290,7 @@ extern "C" int oracle_merge_mark(int n, const float *pos, const float *mass, con
   Grid g{p->cell_length, p->cx, p->cy, p->cz};
   merge_mark(n, reinterpret_cast<const V3<float> *>(pos), mass,
              reinterpret_cast<const V3<float> *>(vel), surf, sd, remove, merge,
-             reinterpret_cast<V3<float> *>(mv), cs, g, max_mass, cand, dens);
+             reinterpret_cast<V3<float> *>(mv), cs, g, max_mass, rules_of(p), cand, dens);
   return 0;
 }
 
@@ -295,8 +309,11 @@ extern "C" int oracle_merge_blend(int n, float *merge, float *mass, float *vel, 
 // removed, stats[1] = split, stats[2] = candidates refused for capacity.
 extern "C" int oracle_split_compact(int n, int cap, float *pos, float *vel, float *mass,
                                     int *adl, int *remove, float *merge, int *mcount,
-                                    const int *surf, const float *bpos, const int *cs,
-                                    const int *bcs, const oracle_params *p, int *stats) {
+                                    const int *surf, const float *sd, const float *bpos,
+                                    const int *cs, const int *bcs, const oracle_params *p,
+                                    int *stats) {
+  const Rules R = rules_of(p);
+  if (R.depth && !sd) return -1;
   Grid g{p->cell_length, p->cx, p->cy, p->cz};
   V3<float> *P = reinterpret_cast<V3<float> *>(pos);
   V3<float> *V = reinterpret_cast<V3<float> *>(vel);
@@ -305,7 +322,7 @@ extern "C" int oracle_split_compact(int n, int cap, float *pos, float *vel, floa
   std::vector<V3<float>> child(n);
 #pragma omp parallel for schedule(dynamic, 256)
   for (int i = 0; i < n; ++i)
-    ok[i] = split_check(i, P, mass, remove, surf, BP, cs, bcs, g, p->density, child[i]) ? 1 : 0;
+    ok[i] = split_check(i, P, mass, remove, surf, sd, BP, cs, bcs, g, R, child[i]) ? 1 : 0;
   int survivors = 0, removed = 0;
   for (int i = 0; i < n; ++i) {
     if (remove[i] != 1) ++survivors;

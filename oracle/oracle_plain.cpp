@@ -775,11 +775,11 @@ extern "C" int oracle_adaptive_steps(int *n_io, int cap, float *pos_, float *vel
     surface_distance(n, pos, surf, cs.data(), g, sd);
     if (step > 0 || merge_first)
       merge_mark(n, pos, mass, vel, surf, sd, remove, merge, mv, cs.data(), g, max_mass,
-                 nullptr, nullptr);
+                 rules_of(p), nullptr, nullptr);
     merge_blend(n, merge, mass, vel, mcount, remove, mv, 40, adl);
     int st[3] = {0, 0, 0};
     *n_io = oracle_split_compact(n, cap, pos_, vel_, mass, adl, remove, merge, mcount, surf,
-                                 bpos_, cs.data(), bcs, p, st);
+                                 sd, bpos_, cs.data(), bcs, p, st);
     if (stats) {
       stats[3 * step] = st[0];
       stats[3 * step + 1] = st[1];

@@ -24,6 +24,8 @@ class oracle_params(C.Structure):
         ("contact_radius_min", C.c_float), ("sleep_speed", C.c_float),
         ("damp_speed", C.c_float), ("damping", C.c_float), ("adaptive_damping", C.c_float),
         ("skin", C.c_float), ("own_only", C.c_int),
+        ("depth_rules", C.c_int), ("min_mass", C.c_float), ("split_mass", C.c_float),
+        ("merge_dist", C.c_float), ("merge_depth", C.c_float), ("split_depth", C.c_float),
     ]
 
 
@@ -85,8 +87,8 @@ def load() -> C.CDLL:
     lib.oracle_merge_mark.argtypes = [C.c_int, fp, fp, fp, ip, fp, ip, fp, fp, ip, P, C.c_float,
                                       vp, vp]
     lib.oracle_merge_blend.argtypes = [C.c_int, fp, fp, fp, ip, ip, fp, C.c_int, ip]
-    lib.oracle_split_compact.argtypes = [C.c_int, C.c_int, fp, fp, fp, ip, ip, fp, ip, ip, fp,
-                                         ip, ip, P, ip]
+    lib.oracle_split_compact.argtypes = [C.c_int, C.c_int, fp, fp, fp, ip, ip, fp, ip, ip, vp,
+                                         fp, ip, ip, P, ip]
     lib.oracle_split_compact.restype = C.c_int
     lib.oracle_adaptive_steps.argtypes = [C.POINTER(C.c_int), C.c_int, fp, fp, fp, ip, ip, fp,
                                           ip, fp, ip, fp, C.c_int, fp, ip, P, C.c_int, C.c_int,
@@ -117,8 +119,10 @@ def load() -> C.CDLL:
 
 
 def params_from(scene, stab=3, solver=5, mu_s=0.0, mu_k=0.0, skin=0.005,
-                own_only=0) -> oracle_params:
-    """Oracle parameters for a scenes.Scene (reference constants elsewhere)."""
+                own_only=0, rules=None) -> oracle_params:
+    """Oracle parameters for a scenes.Scene (reference constants elsewhere).
+    rules: None for the reference merge/split rules, or a dict of
+    depth_rules, min_mass, split_mass, merge_dist, merge_depth, split_depth."""
     p = oracle_params()
     p.dt = float(scene.dt)
     p.gx, p.gy, p.gz = (float(v) for v in scene.gravity)
@@ -137,6 +141,11 @@ def params_from(scene, stab=3, solver=5, mu_s=0.0, mu_k=0.0, skin=0.005,
     p.adaptive_damping = 0.98
     p.skin = skin
     p.own_only = own_only
+    # reference merge / split rules (Solver.cu:526-827)
+    p.depth_rules, p.min_mass, p.split_mass = 0, 1.0, 2.0
+    p.merge_dist, p.merge_depth, p.split_depth = 0.035, 3.0, 2.0
+    for k, v in (rules or {}).items():
+        setattr(p, k, v)
     return p
 
 

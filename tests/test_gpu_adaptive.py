@@ -109,7 +109,7 @@ def test_adaptive_stages_bitexact(gpu):
     MC = np.zeros(cap, np.int32); MC[:n] = mc
     ost = np.zeros(3, np.int32)
     n2 = lib.oracle_split_compact(n, cap, P.reshape(-1), V.reshape(-1), M, A, R, ME, MC, surf,
-                                  np.ascontiguousarray(bpos).reshape(-1), cs,
+                                  None, np.ascontiguousarray(bpos).reshape(-1), cs,
                                   sim.bcs.cpu().numpy(), p, ost)
     _log(f"split stats gpu {list(st)} oracle {list(ost)} n {n} -> {n2}")
     assert sim.set.n == n2 and list(st) == list(ost) and ost[1] > 0

@@ -89,7 +89,11 @@ def test_surface_distance_brute_force(lib):
             assert sd[i] == np.float32(1e5)
 
 
-def _merge(lib, s, pos, surf, sd, remove=None, mass=None, vel=None):
+DEPTH = dict(depth_rules=1, min_mass=0.125, split_mass=0.125, merge_dist=1.75, merge_depth=3.0,
+             split_depth=2.0)
+
+
+def _merge(lib, s, pos, surf, sd, remove=None, mass=None, vel=None, rules=None, max_mass=4.0):
     n = pos.shape[0]
     _, _, cs = po.grid_build(pos, s.cell_length, s.cells)
     mass = np.ones(n, np.float32) if mass is None else np.asarray(mass, np.float32)
@@ -101,8 +105,8 @@ def _merge(lib, s, pos, surf, sd, remove=None, mass=None, vel=None):
     dens = np.zeros(n, np.float32)
     lib.oracle_merge_mark(n, pos.reshape(-1), mass, vel.reshape(-1),
                           np.asarray(surf, np.int32), np.asarray(sd, np.float32), remove, merge,
-                          mv.reshape(-1), cs, po.params_from(s), 4.0, cand.ctypes.data,
-                          dens.ctypes.data)
+                          mv.reshape(-1), cs, po.params_from(s, rules=rules), max_mass,
+                          cand.ctypes.data, dens.ctypes.data)
     return remove, merge, mv, cand, dens
 
 
@@ -176,7 +180,7 @@ def test_blend_conserves_mass_and_removes_donor(lib):
     assert abs(vel[1, 0] - 1.0) < 1e-6
 
 
-def _split(lib, s, pos, mass, surf, remove, bpos=None, cap=None):
+def _split(lib, s, pos, mass, surf, remove, bpos=None, cap=None, sd=None, rules=None):
     n = pos.shape[0]
     cap = cap or 2 * n
     P = np.zeros((cap, 3), np.float32); P[:n] = pos
@@ -193,9 +197,11 @@ def _split(lib, s, pos, mass, surf, remove, bpos=None, cap=None):
     else:
         bpos, bcs = po.sorted_boundary(bpos, s.cell_length, s.cells)
     stats = np.zeros(3, np.int32)
+    sdv = None if sd is None else np.ascontiguousarray(sd, np.float32)
     n2 = lib.oracle_split_compact(n, cap, P.reshape(-1), V.reshape(-1), M, adl, R, mer, mc,
-                                  np.asarray(surf, np.int32), bpos.reshape(-1), cs, bcs,
-                                  po.params_from(s), stats)
+                                  np.asarray(surf, np.int32),
+                                  None if sdv is None else sdv.ctypes.data, bpos.reshape(-1), cs,
+                                  bcs, po.params_from(s, rules=rules), stats)
     return n2, P[:n2], V[:n2], M[:n2], adl[:n2], R[:n2], stats
 
 
@@ -233,6 +239,63 @@ def test_compaction_keeps_order_and_appends_children(lib):
                                      [0, 0, 1, 0], cap=4)
     assert n2 == 4 and list(st) == [1, 1, 1]
     assert list(M) == [1.0, 2.0, 4.0, 2.0]
+
+
+def _r_surface(m):
+    """The surface radius of mass m (float density, double pi, cbrt in
+    double rounded once): what the depth rules scale by."""
+    return np.float32((3.0 * float(np.float32(3.0 * m)) / 3.0 /
+                       (4.0 * math.pi * float(np.float32(238732.4146)))) ** (1.0 / 3.0))
+
+
+def test_depth_rules_merge_below_three_radii(lib):
+    # two unit-mass particles (r = 0.01) 0.02 apart: partners within
+    # 1.75 (r_i + r_j) = 0.035; merging needs both more than 3 r = 0.03
+    # below the surface (sd > (3 r)^2, sd a squared distance)
+    s = _scene([[0.51, 0.51, 0.45], [0.51, 0.51, 0.47]])
+    pos, cs, perm = _sorted(s)
+    r = float(_r_surface(1.0))
+    assert abs(r - 0.01) < 1e-6
+    deep, shallow = (3 * r * 1.01) ** 2, (3 * r * 0.99) ** 2
+    rem, merge, *_ = _merge(lib, s, pos, [0, 0], [deep, deep], rules=DEPTH, max_mass=8.0)
+    assert list(rem) == [-1, -1] and list(merge) == [-1.0, 1.0]
+    rem, merge, *_ = _merge(lib, s, pos, [0, 0], [shallow, deep], rules=DEPTH, max_mass=8.0)
+    assert list(rem) == [0, 0]                     # i too close to the surface
+    rem, merge, *_ = _merge(lib, s, pos, [0, 0], [deep, shallow], rules=DEPTH, max_mass=8.0)
+    assert list(rem) == [0, -1]                    # partner locked (reference quirk kept)
+    # the reference rules would not merge at sd = 0.0009 (< 100)
+    rem, *_ = _merge(lib, s, pos, [0, 0], [deep, deep])
+    assert list(rem) == [0, 0]
+
+
+def test_depth_rules_partner_distance_scales_with_radii(lib):
+    # partners within 1.75 (r_i + r_j): 0.035 for two unit masses, 0.0556 for
+    # two of mass 4 (r = 0.0159); at 0.036 (inside the neighbouring cell)
+    # only the heavier pair merges; the merged mass may reach max_mass
+    # (8 + 1 > 8 refused, 7 + 1 = 8 allowed)
+    for dz, mass, pair in ((0.036, [4.0, 4.0], True), (0.036, [1.0, 1.0], False),
+                           (0.030, [1.0, 1.0], True), (0.030, [8.0, 1.0], False),
+                           (0.030, [7.0, 1.0], True)):
+        s = _scene([[0.51, 0.51, 0.40], [0.51, 0.51, 0.40 + dz]])
+        pos, cs, perm = _sorted(s)
+        ri, rj = (float(_r_surface(m)) for m in mass)
+        assert (np.linalg.norm(pos[0] - pos[1]) < 1.75 * (ri + rj)) == (pair or sum(mass) > 8)
+        rem, *_ = _merge(lib, s, pos, [0, 0], [1e5, 1e5], mass=mass, rules=DEPTH, max_mass=8.0)
+        assert list(rem) == ([-1, -1] if pair else [0, 0]), (dz, mass)
+
+
+def test_depth_rules_split_within_two_radii(lib):
+    # a particle of mass 1 (r = 0.01) splits when it is closer than 2 r to
+    # the surface (sd < (2 r)^2), surface flag or not; mass 0.25 is the
+    # lightest that splits (children of 0.125, the finest level)
+    s = _scene([[0.5, 0.5, 0.5]])
+    r = float(_r_surface(1.0))
+    near, far = (2 * r * 0.99) ** 2, (2 * r * 1.01) ** 2
+    n2, P, V, M, *_ = _split(lib, s, s.pos, [1.0], [0], [0], sd=[near], rules=DEPTH)
+    assert n2 == 2 and list(M) == [0.5, 0.5]
+    assert _split(lib, s, s.pos, [1.0], [1], [0], sd=[far], rules=DEPTH)[0] == 1
+    assert _split(lib, s, s.pos, [0.25], [1], [0], sd=[0.0], rules=DEPTH)[0] == 2
+    assert _split(lib, s, s.pos, [0.125], [1], [0], sd=[0.0], rules=DEPTH)[0] == 1
 
 
 def test_adaptive_steps_conserve_mass():
