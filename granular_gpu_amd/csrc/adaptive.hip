@@ -802,6 +802,8 @@ extern "C" int gg_adaptive_sampling(gg_particles *set, gg_adaptive *ad, const in
   }
   if (do_merge) GG_TRY(merge_mark_rules(set, ad, cell_start, max_mass, R, rs, ws, stream));
   GG_TRY(gg_merge_blend(set, ad, blend_steps, ws, stream));
+  // the split rule reads the radii of the masses after the blend
+  if (R.depth && set->n > 0) GG_TRY(merge_radii(set, params->density, ws, stream, &rs));
   return split_compact_rules(set, ad, cell_start, boundary, bcell_start, params, rs, ws, stream,
                              stats);
 }

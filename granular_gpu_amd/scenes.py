@@ -267,6 +267,42 @@ def funnel_pour(n: int = 8000, cone_count: int = 12000, r: float = 0.01, box: fl
         gravity=(0.0, float(np.float32(g)), 0.0), density=DENSITY_R001, radius=r)
 
 
+def funnel_column(n: int = 1_000_000, cone_count: int = 400_000, r: float = 0.01,
+                  cone_height: float = 2.5, outlet: float = 0.1, half_angle_deg: float = 30.0,
+                  gap: float = 0.6, seed: int = 2, dt: float = 0.002,
+                  g: float = -9.8) -> Scene:
+    """Funnel pour with the space sized to the column: a cone_funnel of
+    `cone_count` particles (outlet `gap` above the floor, `cone_height`
+    tall) in BOX walls 0.1 wider than the rim on each side, and n particles
+    of radius r at seeded uniform-random positions in a cylinder of 0.8 of
+    the rim radius from 0.02 above the rim, (2.4 r)^3 of volume each; the
+    space ends 0.1 above the column.
+
+    n = 1M, 400k cone (defaults): space 3.187 x 6.303 x 3.187, 79 x 157 x 79
+    cells of 0.0404 (979,843), a column 3.08 tall."""
+    L = CELL_LENGTH
+    rim = 0.5 * outlet + cone_height * math.tan(math.radians(half_angle_deg))
+    R = 0.8 * rim
+    tall = n * (2.4 * r) ** 3 / (math.pi * R * R)
+    w = 2.0 * rim + 0.2
+    y0 = gap + cone_height + 0.02
+    space = (w, y0 + tall + 0.1, w)
+    centre = (0.5 * w, gap, 0.5 * w)
+    cone = cone_funnel(cone_count, half_angle_deg, outlet, cone_height, centre)
+    u = uniform01(seed, 3 * n).reshape(n, 3)
+    rad = R * np.sqrt(u[:, 0])
+    phi = 2.0 * math.pi * u[:, 1]
+    pos = np.stack([centre[0] + rad * np.cos(phi), y0 + tall * u[:, 2],
+                    centre[2] + rad * np.sin(phi)], 1).astype(np.float32)
+    walls = box_walls(space, L, "box")
+    return Scene(
+        name=f"funnel_column_{n}_cone_{cone_count}", pos=pos,
+        vel=np.zeros((n, 3), np.float32), mass=np.full(n, radius_mass(r), np.float32),
+        boundary=np.concatenate([walls, cone]).astype(np.float32), space=space,
+        cell_length=L, cells=cell_counts(space, L), dt=np.float32(dt),
+        gravity=(0.0, float(np.float32(g)), 0.0), density=DENSITY_R001, radius=r)
+
+
 def bucket_scoop(count: int, depth: float = 0.2, width: float = 0.3, height: float = 0.15,
                  corner=(0.0, 0.0, 0.0)) -> np.ndarray:
     """About `count` boundary particles on an open scoop: a bottom plate

@@ -143,3 +143,35 @@ def test_adaptive_steps_bitexact(gpu, side, steps):
     for k, ref in (("pos", st.pos), ("vel", st.vel), ("mass", st.mass), ("adl", st.adl),
                    ("remove", st.remove), ("merge", st.merge), ("mcount", st.mcount)):
         assert np.array_equal(got[k], ref[:n]), k
+
+
+DEPTH = dict(min_mass=0.125, split_mass=0.125, merge_dist=1.75, merge_depth=3.0, split_depth=2.0)
+
+
+def test_depth_rules_steps_bitexact_97k(gpu):
+    # radius-level rules on a 46^3 block (97,336 particles) falling onto the
+    # floor: deep particles merge up to mass 8, particles within two radii of
+    # the surface split down to 0.125; whole adaptive steps against the oracle
+    steps = 60
+    _log(f"depth rules side=46 steps={steps}")
+    s = scenes.plain_box(46, 1.0, dt=0.002, g=-9.8)
+    sim = _sim(s, stab=3, solver=5, capacity=4 * s.n, max_mass=8.0)
+    sim.params.adaptive_rules = 1
+    for k, v in DEPTH.items():
+        setattr(sim.params, k, v)
+    sim.step(steps)
+    got = sim.host_all()
+    p = po.params_from(s, 3, 5, rules=dict(DEPTH, depth_rules=1))
+    bpos, bcs = po.sorted_boundary(s.boundary, s.cell_length, s.cells)
+    st = po.AdaptiveState(s.pos, s.vel, s.mass, cap=sim.cap)
+    stats = po.adaptive_steps(st, bpos, bcs, p, steps, max_mass=8.0)
+    _log(f"merged {stats[:, 0].sum()} split {stats[:, 1].sum()} final n {st.n} (gpu {sim.n})")
+    assert stats[:, 0].sum() > 0 and stats[:, 1].sum() > 0
+    assert np.array_equal(np.array(sim.stats, np.int32), stats)
+    assert sim.n == st.n
+    n = st.n
+    for k, ref in (("pos", st.pos), ("vel", st.vel), ("mass", st.mass), ("adl", st.adl),
+                   ("remove", st.remove), ("merge", st.merge), ("mcount", st.mcount)):
+        assert np.array_equal(got[k], ref[:n]), k
+    # mass is conserved up to float rounding of the halvings and blends
+    assert abs(float(st.mass[:n].sum(dtype=np.float64)) - float(s.mass.sum())) < 1e-3 * s.n
