@@ -157,6 +157,8 @@ _PROTOS = {
                                   P(gg_mlp_config), VP, VP, VP, VP]),
     "gg_mlp_teacher": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, gg_grid,
                                  P(gg_upsample_params), P(gg_mlp_config), VP, VP, VP, VP, VP]),
+    "gg_mlp_fine_targets": (C.c_int, [P(gg_particles), P(gg_particles), VP, gg_grid,
+                                      P(gg_mlp_config), VP, VP, VP, VP]),
     "gg_mlp_train_scratch_bytes": (C.c_size_t, [C.c_int]),
     "gg_mlp_forward": (C.c_int, [P(gg_mlp_weights), VP, C.c_int, VP, VP, C.c_size_t, VP]),
     "gg_mlp_grad": (C.c_int, [P(gg_mlp_weights), VP, VP, C.c_int, VP, VP, VP, VP, C.c_size_t,

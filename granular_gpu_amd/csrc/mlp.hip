@@ -352,6 +352,49 @@ __global__ void __launch_bounds__(256)
                  fine_pos, fine_vel);
 }
 
+// Fine-run targets: one thread per (coarse particle, slot).  The fine
+// particle nearest to the template point among the 27 fine cells around it,
+// ties to the lower index (so the visiting order does not matter).
+__global__ void __launch_bounds__(256)
+    k_mlp_fine_targets(int n, const float *__restrict__ pos, const float *__restrict__ vel,
+                       const float *__restrict__ fpos, const float *__restrict__ fvel,
+                       const int *__restrict__ fcs, DevGrid fg, float radius,
+                       float *__restrict__ y, float *__restrict__ d2, int *__restrict__ jm) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8LL * n) return;
+  const int i = (int)(t >> 3), k = (int)(t & 7);
+  const float qx = pos[3 * i] + radius * templ(k, 0);
+  const float qy = pos[3 * i + 1] + radius * templ(k, 1);
+  const float qz = pos[3 * i + 2] + radius * templ(k, 2);
+  float best = INFINITY;
+  int bj = -1;
+  int bx, by, bz;
+  if (floor_cell(qx, qy, qz, fg, bx, by, bz)) {
+    stencil_rows(bx, by, bz, fg, fcs, [&](int j0, int j1) {
+      for (int j = j0; j < j1; ++j) {
+        const Packed3 f = ld3(fpos, j);
+        const float dx = f.x - qx, dy = f.y - qy, dz = f.z - qz;
+        const float dd = dx * dx + dy * dy + dz * dz;
+        if (dd < best || (dd == best && j < bj)) {
+          best = dd;
+          bj = j;
+        }
+      }
+    });
+  }
+  const size_t o = (size_t)i * kOut + 3 * k;
+  if (bj < 0) {
+    y[o] = y[o + 1] = y[o + 2] = 0.0f;
+  } else {
+    const Packed3 w = ld3(fvel, bj);
+    y[o] = w.x - vel[3 * i];
+    y[o + 1] = w.y - vel[3 * i + 1];
+    y[o + 2] = w.z - vel[3 * i + 2];
+  }
+  if (d2) d2[t] = best;
+  if (jm) jm[t] = bj;
+}
+
 __global__ void k_zero_int(int *p) { *p = 0; }
 
 __global__ void __launch_bounds__(256)
@@ -808,6 +851,27 @@ extern "C" int gg_mlp_teacher(const gg_particles *coarse, const int *cs,
       reinterpret_cast<const float *>(coarse->vel), cs,
       reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), up_const(u),
       c->radius, flags, y, fine_pos, fine_vel);
+  GG_LAUNCH_CHECK();
+  return GG_OK;
+}
+
+extern "C" int gg_mlp_fine_targets(const gg_particles *coarse, const gg_particles *fine,
+                                   const int *fine_cell_start, gg_grid fine_grid,
+                                   const gg_mlp_config *c, float *y, float *d2, int *match,
+                                   hipStream_t stream) {
+  if (!coarse || !fine || !fine_cell_start || !c || !y) return GG_ERR_INVALID;
+  if (coarse->n < 0 || fine->n < 0) return GG_ERR_INVALID;
+  if (fine_grid.cx <= 0 || fine_grid.cy <= 0 || fine_grid.cz <= 0 ||
+      !(fine_grid.cell_length > 0.0f) || !(c->radius > 0.0f))
+    return GG_ERR_INVALID;
+  if (coarse->n == 0) return GG_OK;
+  if (!coarse->pos || !coarse->vel || (fine->n > 0 && (!fine->pos || !fine->vel)))
+    return GG_ERR_INVALID;
+  k_mlp_fine_targets<<<nblocks(8LL * coarse->n, 256), 256, 0, stream>>>(
+      coarse->n, reinterpret_cast<const float *>(coarse->pos),
+      reinterpret_cast<const float *>(coarse->vel), reinterpret_cast<const float *>(fine->pos),
+      reinterpret_cast<const float *>(fine->vel), fine_cell_start, dev_grid(fine_grid),
+      c->radius, y, d2, match);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }

@@ -101,11 +101,14 @@ class Trainer:
         return self.dW, self.db, self.sq[:B]
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
-        B = x.shape[0]
-        y = torch.empty((B, 24), dtype=torch.float32, device=x.device)
-        check("gg_mlp_forward", self.lib.gg_mlp_forward(
-            C.byref(self.net.struct), _ptr(x), B, _ptr(y), _ptr(self.scratch),
-            self.scratch.numel(), _stream()))
+        """The training forward over x, in chunks of the trainer's batch."""
+        n = x.shape[0]
+        y = torch.empty((n, 24), dtype=torch.float32, device=x.device)
+        for a in range(0, n, self.B):
+            b = min(n, a + self.B)
+            check("gg_mlp_forward", self.lib.gg_mlp_forward(
+                C.byref(self.net.struct), _ptr(x[a:b]), b - a, _ptr(y[a:b]),
+                _ptr(self.scratch), self.scratch.numel(), _stream()))
         return y
 
     def adam(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
@@ -203,6 +206,48 @@ class LearnedUpsampler:
             xs.append(feat[keep])
             ts.append(y[keep])
         return torch.cat(xs), torch.cat(ts)
+
+
+def fine_targets(coarse_sim, fine_sim, cfg=None):
+    """Training targets of every coarse particle from the fine run
+    (gg_mlp_fine_targets): y (n x 24), d2 (n x 8), match (n x 8).  Both
+    sims must have current cell tables (grid_build after their last step)."""
+    cfg = cfg or default_config()
+    n = coarse_sim.n
+    dev = coarse_sim.pos.device
+    y = torch.empty((n, 24), dtype=torch.float32, device=dev)
+    d2 = torch.empty((n, 8), dtype=torch.float32, device=dev)
+    jm = torch.empty((n, 8), dtype=torch.int32, device=dev)
+    check("gg_mlp_fine_targets", load().gg_mlp_fine_targets(
+        C.byref(coarse_sim.set), C.byref(fine_sim.set), _ptr(fine_sim.cs), fine_sim.grid,
+        C.byref(cfg), _ptr(y), _ptr(d2), _ptr(jm), _stream()))
+    return y, d2, jm
+
+
+def collect_pairs(up: "LearnedUpsampler", coarse_sim, fine_sim, snapshots, every, warmup=0,
+                  max_match=1.0):
+    """Training pairs from a coarse run and its fine twin stepped side by
+    side (scenes.paired_boxes): after `warmup` steps, every `every` steps,
+    the coarse features and the fine-run targets of the coarse particles
+    that are not flagged for fallback and whose 8 matched fine particles all
+    lie within max_match coarse radii of their template points (where the
+    fine run has not drifted away from the coarse one).  Returns (x, t)."""
+    xs, ts = [], []
+    lim = (max_match * up.cfg.radius) ** 2
+    for _ in range(warmup):
+        coarse_sim.step()
+        fine_sim.step()
+    for _ in range(snapshots):
+        coarse_sim.step(every)
+        fine_sim.step(every)
+        coarse_sim.grid_build()
+        fine_sim.grid_build()
+        feat, flags, _ = up.features(coarse_sim)
+        y, d2, _ = fine_targets(coarse_sim, fine_sim, up.cfg)
+        keep = (flags == 0) & (d2 < lim).all(dim=1)
+        xs.append(feat[keep])
+        ts.append(y[keep])
+    return torch.cat(xs), torch.cat(ts)
 
 
 def fine_error(a: torch.Tensor, b: torch.Tensor) -> float:

@@ -303,6 +303,87 @@ def funnel_column(n: int = 1_000_000, cone_count: int = 400_000, r: float = 0.01
         gravity=(0.0, float(np.float32(g)), 0.0), density=DENSITY_R001, radius=r)
 
 
+TEMPLATE8 = np.array([[(k >> 2) & 1, (k >> 1) & 1, k & 1] for k in range(8)],
+                     np.float64) - 0.5
+
+
+def paired_boxes(n_side: int, box: float, r: float = 0.02, seed: int = 1,
+                 dt: float = 0.002, g: float = -9.8) -> tuple[Scene, Scene]:
+    """A coarse plain box (plain_box(n_side, box, seed, r)) and its fine
+    twin seeded identically: every coarse particle i is replaced by the 8
+    particles of radius r/2 at x_i + r T_k (T_k the corners of the unit cube
+    centred at 0, k = 4 sx + 2 sy + sz), fine index 8 i + k, which tile the
+    coarse particle's cube exactly.  Same walls, cells, dt and gravity; the
+    two are stepped side by side to give the learned upsampler's training
+    pairs."""
+    coarse = plain_box(n_side, box, seed=seed, r=r, dt=dt, g=g)
+    fpos = (coarse.pos.astype(np.float64)[:, None, :] + r * TEMPLATE8[None]).reshape(-1, 3)
+    nf = fpos.shape[0]
+    fine = Scene(name=f"{coarse.name}_fine", pos=fpos.astype(np.float32),
+                 vel=np.zeros((nf, 3), np.float32),
+                 mass=np.full(nf, radius_mass(0.5 * r), np.float32),
+                 boundary=coarse.boundary, space=coarse.space, cell_length=coarse.cell_length,
+                 cells=coarse.cells, dt=coarse.dt, gravity=coarse.gravity,
+                 density=coarse.density, radius=0.5 * r)
+    return coarse, fine
+
+
+def disk_heap(nx: int = 16, ny: int = 40, r: float = 0.005, disk_radius: float = 0.12,
+              disk_y: float = 0.1, box: float = 0.6, seed: int = 3, dt: float = 0.002,
+              g: float = -9.8) -> Scene:
+    """A fixed-base heap: an nx x ny x nx jittered lattice (spacing 2r,
+    jitter U(+-0.1r)) of radius-r particles standing on a horizontal disk of
+    boundary particles (radius disk_radius at height disk_y, a square grid
+    of spacing r clipped to the disk), centred in a box of BOX walls with
+    cells of 4.04 r.  The column collapses, the excess falls off the disk's
+    rim and the heap left on the disk stands at the angle of repose."""
+    L = np.float32(4.04 * r)
+    space = (box, box, box)
+    c = 0.5 * box
+    k = int(math.floor(disk_radius / r))
+    gx, gz = np.meshgrid(np.arange(-k, k + 1), np.arange(-k, k + 1), indexing="ij")
+    keep = (gx * gx + gz * gz) * r * r <= disk_radius * disk_radius
+    disk = np.stack([c + r * gx[keep], np.full(int(keep.sum()), disk_y), c + r * gz[keep]], 1)
+    sp = 2.0 * r
+    x0 = c - 0.5 * sp * (nx - 1)
+    y0 = disk_y + 0.01 + r + 0.1 * r
+    pos = jittered_lattice(nx, ny, nx, sp, (x0, y0, x0), 0.1 * r, seed)
+    walls = box_walls(space, L, "box")
+    n = pos.shape[0]
+    return Scene(
+        name=f"disk_heap_{n}", pos=pos, vel=np.zeros((n, 3), np.float32),
+        mass=np.full(n, radius_mass(r), np.float32),
+        boundary=np.concatenate([walls, disk]).astype(np.float32), space=space, cell_length=L,
+        cells=cell_counts(space, L), dt=np.float32(dt),
+        gravity=(0.0, float(np.float32(g)), 0.0), density=DENSITY_R001, radius=r)
+
+
+def heap_angle(pos: np.ndarray, centre, base_y: float, base_radius: float,
+               bins: int = 12) -> float:
+    """Angle of repose (degrees) of the heap on a disk of base_radius whose
+    top surface is at base_y, centred at centre = (x, z): the particles above
+    the disk and inside its rim, binned by distance from the axis; the
+    heap's surface height in a bin is the 90th percentile of the particle
+    heights there, and the angle is that of the least-squares line through
+    the bins between 0.2 and 0.9 of the radius."""
+    p = np.asarray(pos, np.float64)
+    rr = np.hypot(p[:, 0] - centre[0], p[:, 2] - centre[1])
+    h = p[:, 1] - base_y
+    on = (rr < base_radius) & (h > 0.0)
+    rr, h = rr[on], h[on]
+    edges = np.linspace(0.0, base_radius, bins + 1)
+    rb, hb = [], []
+    for a, b in zip(edges[:-1], edges[1:]):
+        m = (rr >= a) & (rr < b)
+        if m.sum() >= 5 and 0.2 * base_radius <= 0.5 * (a + b) <= 0.9 * base_radius:
+            rb.append(0.5 * (a + b))
+            hb.append(np.quantile(h[m], 0.9))
+    if len(rb) < 3:
+        raise ValueError("heap_angle: too few populated bins")
+    slope = np.polyfit(np.array(rb), np.array(hb), 1)[0]
+    return math.degrees(math.atan(-slope))
+
+
 def bucket_scoop(count: int, depth: float = 0.2, width: float = 0.3, height: float = 0.15,
                  corner=(0.0, 0.0, 0.0)) -> np.ndarray:
     """About `count` boundary particles on an open scoop: a bottom plate

@@ -414,6 +414,17 @@ int gg_mlp_teacher(const gg_particles *coarse, const int *cell_start,
                    const gg_particles *boundary, const int *bcell_start, gg_grid grid,
                    const gg_upsample_params *u, const gg_mlp_config *c, const int *flags,
                    float *y, float *fine_pos, float *fine_vel, hipStream_t stream);
+/* Training targets from a fine plain run beside the coarse one (the two
+ * seeded identically, stepped side by side; no reference counterpart).  For
+ * coarse particle i and slot k: the fine particle j nearest to the template
+ * point x_i + radius T_k among the fine particles of the 27 fine cells
+ * around floor(point / L_f), ties to the lower index; y (n x 24) = v_j - v_i,
+ * d2 (n x 8, optional) its squared distance, match (n x 8, optional) = j; no
+ * candidate: y = 0, d2 = +inf, match = -1.  The fine set must be sorted by
+ * fine_cell_start (gg_grid_build over fine_grid). */
+int gg_mlp_fine_targets(const gg_particles *coarse, const gg_particles *fine,
+                        const int *fine_cell_start, gg_grid fine_grid, const gg_mlp_config *c,
+                        float *y, float *d2, int *match, hipStream_t stream);
 /* Training path (MFMA GEMMs, scratch of gg_mlp_train_scratch_bytes(B)). */
 size_t gg_mlp_train_scratch_bytes(int batch);
 int gg_mlp_forward(const gg_mlp_weights *w, const uint16_t *x, int batch, float *y,

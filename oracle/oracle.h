@@ -228,6 +228,9 @@ int oracle_mlp_teacher(int n, const float *pos, const float *vel, const int *cs,
                        const float *bpos, const int *bcs, const oracle_params *p,
                        const oracle_upsample_params *u, const oracle_mlp_cfg *c, float *y,
                        float *fine);
+int oracle_mlp_fine_targets(int n, const float *pos, const float *vel, const float *fpos,
+                            const float *fvel, const int *fcs, float fL, int fcx, int fcy,
+                            int fcz, float radius, float *y, float *d2, int *jm);
 int oracle_mlp_positions(int n, const float *pos, const float *vel, const float *y,
                          const oracle_mlp_cfg *c, float dt, float *fine);
 int oracle_mlp_forward(int B, const float *x, const float *W, const float *b, float *y,

@@ -199,6 +199,60 @@ extern "C" int oracle_mlp_teacher(int n, const float *pos_, const float *vel_, c
   return 0;
 }
 
+// Fine-run targets (the learned upsampler's training pairs from a coarse and
+// a fine plain run, no reference counterpart): for coarse particle i and
+// template slot k the fine particle j nearest to x_i + radius T_k among the
+// fine particles of the 27 fine cells around floor(point / L_f), ties to the
+// lower index; y = v_fine(j) - v_i, d2 = its squared distance, jm = j; no
+// candidate: y = 0, d2 = +inf, jm = -1.  fcs is the fine cell table over
+// the fine grid (fg_* its cell length and counts).
+extern "C" int oracle_mlp_fine_targets(int n, const float *pos_, const float *vel_,
+                                       const float *fpos_, const float *fvel_, const int *fcs,
+                                       float fL, int fcx, int fcy, int fcz, float radius,
+                                       float *y, float *d2, int *jm) {
+  Grid g{fL, fcx, fcy, fcz};
+  const V3<float> *pos = reinterpret_cast<const V3<float> *>(pos_);
+  const V3<float> *vel = reinterpret_cast<const V3<float> *>(vel_);
+  const V3<float> *fpos = reinterpret_cast<const V3<float> *>(fpos_);
+  const V3<float> *fvel = reinterpret_cast<const V3<float> *>(fvel_);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    const V3<float> xi = pos[i], vi = vel[i];
+    for (int k = 0; k < 8; ++k) {
+      const V3<float> t = templ(k);
+      const V3<float> q{xi.x + radius * t.x, xi.y + radius * t.y, xi.z + radius * t.z};
+      float best = INFINITY;
+      int bj = -1;
+      int bx, by, bz;
+      if (floor_cell<float>(q, g.L, bx, by, bz)) {
+        for (int m = 0; m < 27; ++m) {
+          const int cell = stencil_cell(bx, by, bz, m, g);
+          if (cell < 0) continue;
+          for (int j = fcs[cell]; j < fcs[cell + 1]; ++j) {
+            const float dx = fpos[j].x - q.x, dy = fpos[j].y - q.y, dz = fpos[j].z - q.z;
+            const float dd = dx * dx + dy * dy + dz * dz;
+            if (dd < best || (dd == best && j < bj)) {
+              best = dd;
+              bj = j;
+            }
+          }
+        }
+      }
+      float *o = y + (size_t)i * kOut + 3 * k;
+      if (bj < 0) {
+        o[0] = o[1] = o[2] = 0.0f;
+      } else {
+        o[0] = fvel[bj].x - vi.x;
+        o[1] = fvel[bj].y - vi.y;
+        o[2] = fvel[bj].z - vi.z;
+      }
+      if (d2) d2[(size_t)i * 8 + k] = best;
+      if (jm) jm[(size_t)i * 8 + k] = bj;
+    }
+  }
+  return 0;
+}
+
 extern "C" int oracle_mlp_positions(int n, const float *pos_, const float *vel_, const float *y,
                                     const oracle_mlp_cfg *c, float dt, float *fine) {
   const V3<float> *pos = reinterpret_cast<const V3<float> *>(pos_);

@@ -312,6 +312,9 @@ def _mlp_protos(lib):
                                        C.POINTER(oracle_mlp_cfg), C.c_void_p, C.c_void_p]
     lib.oracle_mlp_positions.argtypes = [C.c_int, f32, f32, f32, C.POINTER(oracle_mlp_cfg),
                                          C.c_float, f32]
+    lib.oracle_mlp_fine_targets.argtypes = [C.c_int, f32, f32, f32, f32, i32, C.c_float,
+                                            C.c_int, C.c_int, C.c_int, C.c_float, f32, f32,
+                                            i32]
     lib.oracle_mlp_forward.argtypes = [C.c_int, f32, f32, f32, f32, C.c_int]
     lib.oracle_mlp_backward.argtypes = [C.c_int, f32, f32, f32, f32, f32, f32, C.c_int]
     lib.oracle_mlp_backward.restype = C.c_double
@@ -349,6 +352,23 @@ def mlp_teacher(pos, vel, cs, bpos_sorted, bcs, params, cfg, u=None):
                            _f(bpos_sorted).reshape(-1), bcs, C.byref(params), C.byref(u),
                            C.byref(cfg), y.ctypes.data, fine.ctypes.data)
     return y, fine
+
+
+def mlp_fine_targets(pos, vel, fpos_sorted, fvel_sorted, fcs, f_cell_length, f_cells, radius):
+    """Fine-run targets of every coarse particle: (y n x 24, d2 n x 8,
+    matched fine index n x 8) over the fine set sorted by its cell table."""
+    lib = load()
+    _mlp_protos(lib)
+    n = pos.shape[0]
+    y = np.zeros((n, 24), np.float32)
+    d2 = np.zeros((n, 8), np.float32)
+    jm = np.zeros((n, 8), np.int32)
+    lib.oracle_mlp_fine_targets(n, _f(pos).reshape(-1), _f(vel).reshape(-1),
+                                _f(fpos_sorted).reshape(-1), _f(fvel_sorted).reshape(-1),
+                                np.ascontiguousarray(fcs, np.int32), float(f_cell_length),
+                                *[int(c) for c in f_cells], float(radius), y.reshape(-1),
+                                d2.reshape(-1), jm.reshape(-1))
+    return y, d2, jm
 
 
 def mlp_positions(pos, vel, y, cfg, dt=0.002):

@@ -128,6 +128,55 @@ def test_teacher_is_galilean_invariant():
     assert np.abs(y1 - y0).max() < 2e-6
 
 
+def _fine_sorted(fpos, fvel, s):
+    _, perm, fcs = po.grid_build(fpos, s.cell_length, s.cells)
+    return fpos[perm], fvel[perm], fcs, perm
+
+
+def test_fine_targets_of_the_paired_start_are_the_children():
+    # at the start of a paired run the fine particles sit exactly on the
+    # template points: each slot matches its own child at distance 0
+    coarse, fine = scenes.paired_boxes(5, 1.0, r=R)
+    rng = np.random.default_rng(4)
+    cvel = rng.uniform(-1, 1, (coarse.n, 3)).astype(np.float32)
+    fvel = rng.uniform(-1, 1, (fine.n, 3)).astype(np.float32)
+    fp, fv, fcs, perm = _fine_sorted(fine.pos, fvel, fine)
+    y, d2, jm = po.mlp_fine_targets(coarse.pos, cvel, fp, fv, fcs, fine.cell_length,
+                                    fine.cells, R)
+    # the template points in float32 as the oracle forms them
+    q = coarse.pos[:, None, :] + np.float32(R) * scenes.TEMPLATE8.astype(np.float32)[None]
+    assert np.array_equal(q.reshape(-1, 3), fine.pos)  # float64 and float32 forms agree
+    assert (d2 == 0).all()
+    assert np.array_equal(perm[jm], np.arange(fine.n).reshape(-1, 8))
+    assert np.array_equal(y.reshape(-1, 8, 3), fvel.reshape(-1, 8, 3) - cvel[:, None, :])
+
+
+def test_fine_targets_match_brute_force_nearest():
+    s, pos, vel, _ = _cloud(n=150)
+    rng = np.random.default_rng(9)
+    nf = 3000
+    fpos = rng.uniform(0.3, 0.7, (nf, 3)).astype(np.float32)
+    fvel = rng.uniform(-1, 1, (nf, 3)).astype(np.float32)
+    # duplicated fine particles: equal distances resolve to the lower index
+    fpos[1500:1600] = fpos[:100]
+    fp, fv, fcs, _ = _fine_sorted(fpos, fvel, s)
+    y, d2, jm = po.mlp_fine_targets(pos, vel, fp, fv, fcs, s.cell_length, s.cells, R)
+    T = scenes.TEMPLATE8.astype(np.float32)
+    checked = 0
+    for i in range(pos.shape[0]):
+        for k in range(8):
+            q = pos[i] + np.float32(R) * T[k]
+            d = fp - q
+            dd = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+            j = int(np.argmin(dd))  # first of the minima: the lower index
+            if np.sqrt(dd[j]) >= s.cell_length:
+                continue  # the nearest may lie outside the 27-cell stencil
+            checked += 1
+            assert jm[i, k] == j and d2[i, k] == dd[j]
+            assert np.array_equal(y[i, 3 * k:3 * k + 3], fv[j] - vel[i])
+    assert checked > 1000
+
+
 def _net(seed=1, scale=0.15):
     rng = np.random.default_rng(seed)
     W = np.zeros(po.MLP_W, np.float32)
