@@ -91,20 +91,57 @@ def upsample_phase(args, lib, torch, C, np):
 
     run(args.upsample_warmup)
     train = None
+    fine_truth = None
     if learned is not None:
-        # training pairs: 3 snapshots 10 steps apart, 50k pairs drawn from them
-        xs, ts = learned.collect(sim, snapshots=3, every=10)
+        # training pairs: the coarse scene and its fine twin (r = 0.01, 8 per
+        # coarse on the template points, scenes.paired_boxes) run side by
+        # side from step 0; 3 snapshots 10 steps apart after the warm-up,
+        # 50k pairs drawn from them; then scored on the pair 10 steps later
+        from granular_gpu_amd.learned import collect_pairs, fine_targets
+        pc, pf = scenes.paired_boxes(side, box, r=0.02, seed=1, dt=args.dt, g=-9.8)
+        csim_t, fsim_t = PlainSim(pc, stab=3, solver=5), PlainSim(pf, stab=3, solver=5)
+        t0 = time.perf_counter()
+        xs, ts = collect_pairs(learned, csim_t, fsim_t, snapshots=3, every=10,
+                               warmup=args.upsample_warmup)
+        torch.cuda.synchronize()
+        t_pairs = time.perf_counter() - t0
         g = torch.Generator(device="cpu").manual_seed(1)
         pick = torch.randperm(xs.shape[0], generator=g)[:50000].to(xs.device)
+        kept = int(xs.shape[0])
         xs, ts = xs[pick].contiguous(), ts[pick].contiguous()
         tr = Trainer(learned.net, batch=1024)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         losses = tr.fit(xs, ts, steps=args.learned_train_steps, lr=2e-3, lr_final=1e-4, seed=0)
         torch.cuda.synchronize()
-        train = {"pairs": int(xs.shape[0]), "steps": args.learned_train_steps, "batch": 1024,
+        train = {"data": f"coarse {pc.n} (r=0.02) + fine twin {pf.n} (r=0.01) plain runs, "
+                         f"steps {args.upsample_warmup + 10}, {args.upsample_warmup + 20}, "
+                         f"{args.upsample_warmup + 30}; targets = velocity of the fine "
+                         f"particle nearest each template point minus the parent's",
+                 "pairs_available": kept, "pair_generation_s": round(t_pairs, 3),
+                 "pairs": int(xs.shape[0]), "steps": args.learned_train_steps, "batch": 1024,
                  "seconds": round(time.perf_counter() - t0, 3),
                  "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-50:].mean())}
+        # against the fine-run truth 10 steps after the last snapshot
+        csim_t.step(10)
+        fsim_t.step(10)
+        csim_t.grid_build()
+        fsim_t.grid_build()
+        feat, flags, _ = learned.features(csim_t)
+        truth, d2, _ = fine_targets(csim_t, fsim_t, learned.cfg)
+        keep = (flags == 0) & (d2 < (0.02 ** 2)).all(dim=1)
+        yt = truth[keep]
+        y_net = tr.forward(feat[keep].contiguous())
+        y_tch = learned.teacher(csim_t)[0][keep]
+
+        def rms(a):
+            return round(float(torch.sqrt(((a - yt) ** 2).mean()).item()), 5)
+        fine_truth = {"step": args.upsample_warmup + 40, "coarse_scored": int(keep.sum().item()),
+                      "velocity_rms_m_per_s": {"learned": rms(y_net),
+                                               "rigid": rms(torch.zeros_like(yt)),
+                                               "teacher": rms(y_tch)}}
+        del csim_t, fsim_t, feat, flags, truth, d2, y_net, y_tch
+        torch.cuda.empty_cache()
         lpos = torch.empty((sim.n, 24), dtype=torch.float32, device=sim.pos.device)
     torch.cuda.synchronize()
     lib.gg_profiler_enable(args.upsample_steps * 32 + 16)
@@ -178,6 +215,7 @@ def upsample_phase(args, lib, torch, C, np):
         learned_out = {
             "network": "96-128-128-128-24 ReLU, bf16 weights/activations, fp32 accumulate",
             "training": train,
+            "vs_fine_run_truth": fine_truth,
             "fine_particles_per_s": round(nf / (t_all * 1e-6), 1),
             "speedup_vs_plain": round((up_us + sort_us) / t_all, 3),
             "us_per_call": round(t_all, 3),

@@ -85,6 +85,49 @@ struct MlpConst {
 // bit 2: below guard_floor or near the x = 0 / z = 0 planes.  Flagged
 // particles are appended to a compacted list (order is irrelevant: each one
 // writes only its own outputs).
+__device__ __forceinline__ int guard_flag(const Packed3 &xi, const float *__restrict__ bpos,
+                                          const int *__restrict__ bcs, const DevGrid &g,
+                                          const MlpConst &c) {
+  int flag = 0;
+  const float gb2 = c.guard_boundary * c.guard_boundary;
+  int bx, by, bz;
+  if (floor_cell(xi.x, xi.y, xi.z, g, bx, by, bz)) {
+    stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
+      int k = k0;
+      for (; k + 4 <= k1; k += 4) {
+        const Packed3 q0 = ld3(bpos, k), q1 = ld3(bpos, k + 1), q2 = ld3(bpos, k + 2),
+                      q3 = ld3(bpos, k + 3);
+        auto near = [&](const Packed3 &q) {
+          const float dx = q.x - xi.x, dy = q.y - xi.y, dz = q.z - xi.z;
+          return dx * dx + dy * dy + dz * dz < gb2;
+        };
+        if (near(q0) || near(q1) || near(q2) || near(q3)) flag |= 1;
+      }
+      for (; k < k1; ++k) {
+        const Packed3 q = ld3(bpos, k);
+        const float dx = q.x - xi.x, dy = q.y - xi.y, dz = q.z - xi.z;
+        if (dx * dx + dy * dy + dz * dz < gb2) flag |= 1;
+      }
+    });
+  }
+  if (xi.y < c.guard_floor || xi.x < c.guard_side || xi.z < c.guard_side) flag |= 2;
+  return flag;
+}
+
+// Appends the wave's flagged particles to the compacted list.
+__device__ __forceinline__ void list_flagged(int i, int flag, int *__restrict__ fb_list,
+                                             int *__restrict__ fb_count) {
+  const unsigned long long m = __ballot(flag != 0);
+  if (flag) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    int slot = 0;
+    if (lane == leader) slot = atomicAdd(fb_count, __popcll(m));
+    slot = __shfl(slot, leader, 64) + __popcll(m & ((1ull << lane) - 1ull));
+    fb_list[slot] = i;
+  }
+}
+
 __global__ void __launch_bounds__(256)
     k_mlp_flags(int n, const float *__restrict__ pos, const float *__restrict__ bpos,
                 const int *__restrict__ bcs, DevGrid g, MlpConst c, int *__restrict__ flags,
@@ -92,42 +135,10 @@ __global__ void __launch_bounds__(256)
   const int i = xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x;
   int flag = 0;
   if (i < n) {
-    const Packed3 xi = ld3(pos, i);
-    const float gb2 = c.guard_boundary * c.guard_boundary;
-    int bx, by, bz;
-    if (floor_cell(xi.x, xi.y, xi.z, g, bx, by, bz)) {
-      stencil_rows(bx, by, bz, g, bcs, [&](int k0, int k1) {
-        int k = k0;
-        for (; k + 4 <= k1; k += 4) {
-          const Packed3 q0 = ld3(bpos, k), q1 = ld3(bpos, k + 1), q2 = ld3(bpos, k + 2),
-                        q3 = ld3(bpos, k + 3);
-          auto near = [&](const Packed3 &q) {
-            const float dx = q.x - xi.x, dy = q.y - xi.y, dz = q.z - xi.z;
-            return dx * dx + dy * dy + dz * dz < gb2;
-          };
-          if (near(q0) || near(q1) || near(q2) || near(q3)) flag |= 1;
-        }
-        for (; k < k1; ++k) {
-          const Packed3 q = ld3(bpos, k);
-          const float dx = q.x - xi.x, dy = q.y - xi.y, dz = q.z - xi.z;
-          if (dx * dx + dy * dy + dz * dz < gb2) flag |= 1;
-        }
-      });
-    }
-    if (xi.y < c.guard_floor || xi.x < c.guard_side || xi.z < c.guard_side) flag |= 2;
+    flag = guard_flag(ld3(pos, i), bpos, bcs, g, c);
     flags[i] = flag;
   }
-  if (fb_list) {
-    const unsigned long long m = __ballot(flag != 0);
-    if (flag) {
-      const int lane = threadIdx.x & 63;
-      const int leader = __ffsll((long long)m) - 1;
-      int slot = 0;
-      if (lane == leader) slot = atomicAdd(fb_count, __popcll(m));
-      slot = __shfl(slot, leader, 64) + __popcll(m & ((1ull << lane) - 1ull));
-      fb_list[slot] = i;
-    }
-  }
+  if (fb_list) list_flagged(i, flag, fb_list, fb_count);
 }
 
 // ---- features ---------------------------------------------------------------
@@ -187,13 +198,26 @@ __device__ __forceinline__ void kv_merge16(unsigned long long (&top)[16],
 // GROUPED = 1: feat[g][i][4] for the 24 groups g of 4 consecutive features
 // (the inference layout: a wave's stores and the network's B-operand loads
 // are contiguous).
-template <int GROUPED>
+// FLAGS = 1 (inference): also the fallback flags and the compacted list of
+// the flagged particles (k_mlp_flags fused in).
+template <int GROUPED, int FLAGS>
 __global__ void __launch_bounds__(256)
     k_mlp_features(int n, const float *__restrict__ pos, const float *__restrict__ vel,
                    const int *__restrict__ cs, DevGrid g, MlpConst c,
-                   uint16_t *__restrict__ feat, int *__restrict__ knn) {
+                   uint16_t *__restrict__ feat, int *__restrict__ knn,
+                   const float *__restrict__ bpos, const int *__restrict__ bcs,
+                   int *__restrict__ flags, int *__restrict__ fb_list,
+                   int *__restrict__ fb_count) {
   // consecutive blocks stay on one XCD (spatial neighbours share its L2)
   const int i = xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x;
+  if (FLAGS) {
+    int flag = 0;
+    if (i < n) {
+      flag = guard_flag(ld3(pos, i), bpos, bcs, g, c);
+      flags[i] = flag;
+    }
+    list_flagged(i, flag, fb_list, fb_count);
+  }
   if (i >= n) return;
   const Packed3 xi = ld3(pos, i);
   const float xx = xi.x, xy = xi.y, xz = xi.z;
@@ -638,44 +662,65 @@ __device__ __forceinline__ void output_layer(const uint16_t *__restrict__ W,
   }
 }
 
+// Layer-1 operands of sample group grp: k-step ks of lane (c, q) = feature
+// groups 8 ks + q and 8 ks + 4 + q of sample grp * 16 TILES + 16 s + c.
+template <int TILES>
+__device__ __forceinline__ void load_features(int n, const uint2 *__restrict__ feat4, int grp,
+                                              int groups, int c, int q, uint4 (&f)[TILES][3]) {
+#pragma unroll
+  for (int s = 0; s < TILES; ++s) {
+    const int smp = grp * 16 * TILES + 16 * s + c;
+#pragma unroll
+    for (int ks = 0; ks < 3; ++ks) {
+      uint2 lo = make_uint2(0u, 0u), hi = make_uint2(0u, 0u);
+      if (grp < groups && smp < n) {
+        lo = feat4[(size_t)(8 * ks + q) * n + smp];
+        hi = feat4[(size_t)(8 * ks + 4 + q) * n + smp];
+      }
+      f[s][ks] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+    }
+  }
+}
+
 // feat4: grouped features, feat4[g * n + i] = features 4g..4g+3 of sample i.
+// Each wave loads the next group's features before computing the current
+// one, so their latency hides behind the network.  vec4: fine_pos and
+// fine_vel are 16-byte aligned, and each lane stores its 4 consecutive
+// outputs of a sample (16t + 4q .. + 3) as one 16-byte store.
 template <int WAVES, int TILES>
 __global__ void __launch_bounds__(64 * WAVES)
     k_mlp_infer(int n, const uint2 *__restrict__ feat4, const int *__restrict__ flags,
                 const float *__restrict__ pos, const float *__restrict__ vel,
                 const uint16_t *__restrict__ wts, const float *__restrict__ bias_g, float radius,
-                float dt, float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
+                float dt, float *__restrict__ fine_pos, float *__restrict__ fine_vel,
+                int vec4) {
   __shared__ __attribute__((aligned(16))) uint16_t W[49152];
   __shared__ __attribute__((aligned(16))) float bias[416];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = lane & 15, q = lane >> 4;
+  const int per_iter = 16 * TILES;
+  const int groups = (n + per_iter - 1) / per_iter;
+  const int stride = gridDim.x * WAVES;
+  int grp = blockIdx.x * WAVES + wave;
+  uint4 cur[TILES][3];
+  load_features<TILES>(n, feat4, grp, groups, c, q, cur);
   stage_layer(W + kWOff1, wts + kWOff1, kHid, kIn / 32);
   stage_layer(W + kWOff2, wts + kWOff2, kHid, kHid / 32);
   stage_layer(W + kWOff3, wts + kWOff3, kHid, kHid / 32);
   stage_layer(W + kWOff4, wts + kWOff4, kOutPad, kHid / 32);
   for (int i = threadIdx.x; i < 416; i += blockDim.x) bias[i] = bias_g[i];
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c = lane & 15, q = lane >> 4;
-  const int per_iter = 16 * TILES;
-  const int groups = (n + per_iter - 1) / per_iter;
-  for (int grp = blockIdx.x * WAVES + wave; grp < groups; grp += gridDim.x * WAVES) {
+  for (; grp < groups; grp += stride) {
     const int base = grp * per_iter;
+    uint4 nxt[TILES][3];
+    load_features<TILES>(n, feat4, grp + stride, groups, c, q, nxt);
     bf16x8 in[TILES][4];
     bf16x4 h[TILES][8];
     f32x4 y[TILES][2];
-    // layer 1 operands: k-step ks of lane (c, q) = feature groups 8 ks + q
-    // and 8 ks + 4 + q of sample base + 16 s + c
 #pragma unroll
     for (int s = 0; s < TILES; ++s) {
-      const int smp = base + 16 * s + c;
 #pragma unroll
-      for (int ks = 0; ks < 3; ++ks) {
-        uint2 lo = make_uint2(0u, 0u), hi = make_uint2(0u, 0u);
-        if (smp < n) {
-          lo = feat4[(size_t)(8 * ks + q) * n + smp];
-          hi = feat4[(size_t)(8 * ks + 4 + q) * n + smp];
-        }
-        in[s][ks] = __builtin_bit_cast(bf16x8, make_uint4(lo.x, lo.y, hi.x, hi.y));
-      }
+      for (int ks = 0; ks < 3; ++ks) in[s][ks] = __builtin_bit_cast(bf16x8, cur[s][ks]);
       in[s][3] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
     }
     hidden_layer<TILES, 3>(W + kWOff1, bias + kBOff1, in, h, lane);
@@ -693,21 +738,36 @@ __global__ void __launch_bounds__(64 * WAVES)
       const float px = pos[3 * smp], py = pos[3 * smp + 1], pz = pos[3 * smp + 2];
       const float vx = vel[3 * smp], vy = vel[3 * smp + 1], vz = vel[3 * smp + 2];
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
+      for (int t = 0; t < 2; ++t) {
+        if (16 * t + 4 * q >= kOut) continue;
+        float fp[4], fv[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int f = 16 * t + 4 * q + e;
-          if (f >= kOut) continue;
           const int k = f / 3, ax = f % 3;
-          const float yv = y[s][t][e];
           const float p0 = ax == 0 ? px : (ax == 1 ? py : pz);
           const float v0 = ax == 0 ? vx : (ax == 1 ? vy : vz);
-          const float vf = v0 + yv;
-          const size_t o = (size_t)smp * kOut + f;
-          fine_pos[o] = (p0 + radius * templ(k, ax)) + dt * vf;
-          if (fine_vel) fine_vel[o] = vf;
+          fv[e] = v0 + y[s][t][e];
+          fp[e] = (p0 + radius * templ(k, ax)) + dt * fv[e];
         }
+        const size_t o = (size_t)smp * kOut + 16 * t + 4 * q;
+        if (vec4) {
+          *reinterpret_cast<float4 *>(fine_pos + o) = make_float4(fp[0], fp[1], fp[2], fp[3]);
+          if (fine_vel)
+            *reinterpret_cast<float4 *>(fine_vel + o) = make_float4(fv[0], fv[1], fv[2], fv[3]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            fine_pos[o + e] = fp[e];
+            if (fine_vel) fine_vel[o + e] = fv[e];
+          }
+        }
+      }
     }
+#pragma unroll
+    for (int s = 0; s < TILES; ++s)
+#pragma unroll
+      for (int ks = 0; ks < 3; ++ks) cur[s][ks] = nxt[s][ks];
   }
 }
 
@@ -830,9 +890,10 @@ extern "C" int gg_mlp_features(const gg_particles *coarse, const int *cs,
       reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), mlp_const(c), flags,
       nullptr, nullptr);
   GG_LAUNCH_CHECK();
-  k_mlp_features<0><<<nblocks(coarse->n, 256), 256, 0, stream>>>(
+  k_mlp_features<0, 0><<<nblocks(coarse->n, 256), 256, 0, stream>>>(
       coarse->n, reinterpret_cast<const float *>(coarse->pos),
-      reinterpret_cast<const float *>(coarse->vel), cs, dev_grid(grid), mlp_const(c), feat, knn);
+      reinterpret_cast<const float *>(coarse->vel), cs, dev_grid(grid), mlp_const(c), feat, knn,
+      nullptr, nullptr, nullptr, nullptr, nullptr);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
@@ -967,8 +1028,9 @@ static int launch_infer(int n, const uint2 *feat4, const int *flags, const float
   const int groups = (n + 16 * TILES - 1) / (16 * TILES);
   int blocks = (groups + WAVES - 1) / WAVES;
   if (blocks > cus) blocks = cus;
+  const int vec4 = ((uintptr_t)fine_pos % 16 == 0 && (uintptr_t)fine_vel % 16 == 0) ? 1 : 0;
   k_mlp_infer<WAVES, TILES><<<blocks, 64 * WAVES, 0, st>>>(n, feat4, flags, pos, vel, w, bias,
-                                                          radius, dt, fine_pos, fine_vel);
+                                                          radius, dt, fine_pos, fine_vel, vec4);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
@@ -1032,15 +1094,15 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const uint2 *feat4 = reinterpret_cast<const uint2 *>(feat);
-  // flags and the compacted fallback list first, so that the teacher on the
-  // flagged particles (the plain upsampler on the template) runs on the side
-  // stream beside the features and the network (they write disjoint outputs)
+  // features with the flags and the compacted fallback list fused in, then
+  // the teacher on the flagged particles (the plain upsampler on the
+  // template) on the side stream beside the network (disjoint outputs)
   prof_begin(GG_STAGE_MLP_FEATURES, stream);
   k_zero_int<<<1, 1, 0, stream>>>(fb_count);
   GG_LAUNCH_CHECK();
-  k_mlp_flags<<<nblocks(n, 256), 256, 0, stream>>>(
-      n, pos, reinterpret_cast<const float *>(boundary->pos), bcs, g, mlp_const(c), flags,
-      fb_list, fb_count);
+  k_mlp_features<1, 1><<<nblocks(n, 256), 256, 0, stream>>>(
+      n, pos, vel, cs, g, mlp_const(c), feat, nullptr,
+      reinterpret_cast<const float *>(boundary->pos), bcs, flags, fb_list, fb_count);
   GG_LAUNCH_CHECK();
   SideStream *side = nullptr;
   GG_TRY(side_stream(side));
@@ -1054,12 +1116,6 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   GG_LAUNCH_CHECK();
   prof_end(GG_STAGE_MLP_FALLBACK, side->st);
   if (hipEventRecord(side->join, side->st) != hipSuccess) return GG_ERR_HIP;
-  k_mlp_features<1><<<nblocks(n, 256), 256, 0, stream>>>(n, pos, vel, cs, g, mlp_const(c), feat,
-                                                         nullptr);
-  if (hipGetLastError() != hipSuccess) {
-    (void)hipStreamWaitEvent(stream, side->join, 0);
-    return GG_ERR_HIP;
-  }
   prof_end(GG_STAGE_MLP_FEATURES, stream);
   prof_begin(GG_STAGE_MLP, stream);
   int rc;
