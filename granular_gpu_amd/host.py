@@ -87,6 +87,11 @@ def load() -> C.CDLL:
         "ggh_system_save_stats": (C.c_int, [vp, C.c_char_p]),
         "ggh_system_set_solver": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float,
                                             C.c_int]),
+        "ggh_system_set_lists": (C.c_int, [vp, C.c_int, C.c_int, C.c_float]),
+        "ggh_system_set_merge_rules": (C.c_int, [vp, C.c_int, C.c_float, C.c_float, C.c_float,
+                                                 C.c_float, C.c_float, C.c_float]),
+        "ggh_system_set_learned": (C.c_int, [vp, vp, vp, C.c_float, C.c_float, C.c_float,
+                                             C.c_float, C.c_float]),
         "ggh_darray_selftest": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
         "ggh_solver_stages_selftest": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
         "ggh_write_vtk": (C.c_int, [C.c_char_p, C.c_char_p, vp, C.c_int]),
@@ -382,3 +387,28 @@ class HostSystem:
     def set_solver(self, stab=3, solver=5, mu_s=0.0, mu_k=0.0, flags=0):
         _chk(self.lib, self.lib.ggh_system_set_solver(self.h, stab, solver, mu_s, mu_k, flags),
              "ggh_system_set_solver")
+
+    def set_lists(self, gmax=48, bmax=16, skin=0.0075):
+        """Solver::set_neighbor_lists: list capacities and skin."""
+        _chk(self.lib, self.lib.ggh_system_set_lists(self.h, int(gmax), int(bmax), float(skin)),
+             "ggh_system_set_lists")
+
+    def set_merge_rules(self, depth_rules=1, min_mass=0.125, split_mass=0.125, merge_dist=1.75,
+                        merge_depth=3.0, split_depth=2.0, max_mass=8.0):
+        """GranularSystem::set_merge_rules (defaults: the radius-level rules
+        for masses 0.125 .. 8)."""
+        _chk(self.lib, self.lib.ggh_system_set_merge_rules(
+            self.h, int(depth_rules), min_mass, split_mass, merge_dist, merge_depth, split_depth,
+            max_mass), "ggh_system_set_merge_rules")
+
+    def set_learned(self, w_bf16: np.ndarray, bias: np.ndarray, radius=0.02, vscale=1.0,
+                    guard_boundary=0.05, guard_floor=0.08, guard_side=0.05):
+        """GranularSystem::set_learned_upsampler: w_bf16 the 49,152 bfloat16
+        bit patterns (uint16), bias the 416 float biases."""
+        w = np.ascontiguousarray(w_bf16, np.uint16)
+        b = np.ascontiguousarray(bias, np.float32)
+        if w.size != 49152 or b.size != 416:
+            raise ValueError("set_learned: 49,152 weights and 416 biases expected")
+        _chk(self.lib, self.lib.ggh_system_set_learned(
+            self.h, w.ctypes.data, b.ctypes.data, radius, vscale, guard_boundary, guard_floor,
+            guard_side), "ggh_system_set_learned")

@@ -126,15 +126,19 @@ float GranularSystem::step() {
     // neighbor_search_granular + Solver::step, fused
     _solver.sort_and_step(_particles, _boundaries, _cell_start_particle, _cell_start_boundary,
                           _space_size, _cell_size, _cell_length, _dt, _g, _density);
-    neighbor_search_upsampled(_upsampled, _cell_start_upsampled);
-    _solver.upsampled_update(_particles, _boundaries, _upsampled, _cell_start_upsampled,
-                             _cell_start_particle, _cell_start_boundary, _cell_size, _space_size,
-                             _cell_length, _density);
+    if (_learned) {
+      learned_update();
+    } else {
+      neighbor_search_upsampled(_upsampled, _cell_start_upsampled);
+      _solver.upsampled_update(_particles, _boundaries, _upsampled, _cell_start_upsampled,
+                               _cell_start_particle, _cell_start_boundary, _cell_size,
+                               _space_size, _cell_length, _density);
+    }
     if (_is_adaptive == 1) {
       set_surface_particles(_particles, _cell_start_particle);
       find_distance_to_surface(_particles, _cell_start_particle);
       _solver.adaptive_sampling(_particles, _boundaries, _cell_start_particle,
-                                _cell_start_boundary, (float)_max_mass, _cell_size, _space_size,
+                                _cell_start_boundary, _merge_max_mass, _cell_size, _space_size,
                                 _cell_length, _density);
     }
   } catch (const char *s) {
@@ -153,4 +157,58 @@ float GranularSystem::step() {
   frame_times.push_back(milliseconds);
   if (_verbose) std::cout << "Frame time : " << milliseconds << "\n";
   return 1;
+}
+
+void GranularSystem::set_merge_rules(int depth_rules, float min_mass, float split_mass,
+                                     float merge_dist, float merge_depth, float split_depth,
+                                     float max_mass) {
+  if (!(min_mass > 0.0f) || !(split_mass > 0.0f) || !(merge_dist > 0.0f) ||
+      !(max_mass > min_mass))
+    throw std::invalid_argument("set_merge_rules: masses and distances must be positive");
+  _solver.set_merge_rules(depth_rules, min_mass, split_mass, merge_dist, merge_depth,
+                          split_depth);
+  _merge_max_mass = max_mass;
+}
+
+void GranularSystem::set_learned_upsampler(const uint16_t *w_bf16, const float *bias,
+                                           const gg_mlp_config &cfg) {
+  if (!w_bf16 || !bias) throw std::invalid_argument("set_learned_upsampler: null weights");
+  if (!(cfg.radius > 0.0f) || !(cfg.vscale > 0.0f))
+    throw std::invalid_argument("set_learned_upsampler: radius and vscale must be positive");
+  if (!_mlp_w) _mlp_w.reset(new DArray<unsigned int>(GG_MLP_WEIGHTS / 2));
+  if (!_mlp_b) _mlp_b.reset(new DArray<float>(GG_MLP_BIASES));
+  HIP_CALL(hipMemcpy(_mlp_w->addr(), w_bf16, sizeof(uint16_t) * GG_MLP_WEIGHTS,
+                     hipMemcpyHostToDevice));
+  HIP_CALL(hipMemcpy(_mlp_b->addr(), bias, sizeof(float) * GG_MLP_BIASES,
+                     hipMemcpyHostToDevice));
+  _mlp_cfg = cfg;
+  _learned = true;
+}
+
+// 8 fine particles per granular particle from the network (features over
+// the step's granular cell table; teacher fallback near boundaries).
+void GranularSystem::learned_update() {
+  const unsigned int n = _particles->size();
+  const unsigned int want = 8u * std::max(_particles->device_capacity(), 1u);
+  if (_upsampled->device_capacity() < want) _upsampled->reserve(want);
+  _upsampled->set_size(8u * n);
+  if (n == 0) return;
+  const size_t need = gg_upsample_mlp_scratch_bytes((int)n);
+  const size_t words = (need + 3) / 4;
+  if (words > 0xffffffffu) throw std::runtime_error("learned_update: scratch too large");
+  if (!_mlp_scratch || _mlp_scratch->length() < words)
+    _mlp_scratch.reset(new DArray<unsigned int>((unsigned int)words));
+  gg_particles c = _particles->view();
+  gg_particles b = _boundaries->view();
+  gg_upsample_params u;
+  gg_upsample_params_default(&u);
+  const gg_grid gr{_cell_length, _cell_size.x, _cell_size.y, _cell_size.z};
+  check(gg_upsample_mlp(&c, _cell_start_particle.addr(), &b, _cell_start_boundary.addr(), gr, &u,
+                        &_mlp_cfg, reinterpret_cast<const uint16_t *>(_mlp_w->addr()),
+                        _mlp_b->addr(),
+                        reinterpret_cast<float *>(_upsampled->get_pos_ptr()),
+                        reinterpret_cast<float *>(_upsampled->get_vel_ptr()),
+                        _mlp_scratch->addr(), 4 * (size_t)_mlp_scratch->length(), nullptr,
+                        nullptr),
+        "gg_upsample_mlp");
 }

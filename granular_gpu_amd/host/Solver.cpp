@@ -12,9 +12,6 @@ void check(int rc, const char *what) {
     throw std::runtime_error(std::string(what) + ": " + gg_status_string(rc));
 }
 
-constexpr int kGmax = 48;  // granular candidates per neighbour list
-constexpr int kBmax = 16;  // boundary candidates per neighbour list
-
 }  // namespace
 
 Solver::Solver(const std::shared_ptr<GranularParticles> &particles)
@@ -36,6 +33,24 @@ void Solver::set_iterations(int stab, int solver) {
 void Solver::set_friction(float mu_s, float mu_k) {
   _mu_s = mu_s;
   _mu_k = mu_k;
+}
+
+void Solver::set_neighbor_lists(int gmax, int bmax, float skin) {
+  if (gmax < 0 || bmax < 0 || !(skin >= 0.0f))
+    throw std::invalid_argument("set_neighbor_lists: negative capacity or skin");
+  _gmax = gmax;
+  _bmax = bmax;
+  _skin = skin;
+}
+
+void Solver::set_merge_rules(int depth_rules, float min_mass, float split_mass,
+                             float merge_dist, float merge_depth, float split_depth) {
+  _rules = depth_rules ? 1 : 0;
+  _min_mass = min_mass;
+  _split_mass = split_mass;
+  _merge_dist = merge_dist;
+  _merge_depth = merge_depth;
+  _split_depth = split_depth;
 }
 
 void Solver::set_flags(int flags) { _flags = flags; }
@@ -60,6 +75,13 @@ gg_params Solver::params(float dt, float3 G, float density) const {
   p.mu_s = _mu_s;
   p.mu_k = _mu_k;
   p.flags = _flags;
+  p.skin = _skin;
+  p.adaptive_rules = _rules;
+  p.min_mass = _min_mass;
+  p.split_mass = _split_mass;
+  p.merge_dist = _merge_dist;
+  p.merge_depth = _merge_depth;
+  p.split_depth = _split_depth;
   return p;
 }
 
@@ -86,15 +108,15 @@ void Solver::ensure_workspace(int cap, int bcap, gg_grid g) {
   const bool same_grid = _ws_ready && _ws.grid.cx == g.cx && _ws.grid.cy == g.cy &&
                          _ws.grid.cz == g.cz && _ws.grid.cell_length == g.cell_length;
   if (same_grid && _ws.capacity >= cap && _ws.boundary_capacity >= bcap &&
-      _ws.max_sweeps >= sweeps)
+      _ws.max_sweeps >= sweeps && _ws.gmax == _gmax && _ws.bmax == _bmax)
     return;
   gg_workspace w{};
   w.capacity = same_grid ? std::max(cap, _ws.capacity) : cap;
   w.boundary_capacity = same_grid ? std::max(bcap, _ws.boundary_capacity) : bcap;
   w.grid = g;
   w.max_sweeps = same_grid ? std::max(sweeps, _ws.max_sweeps) : sweeps;
-  w.gmax = kGmax;
-  w.bmax = kBmax;
+  w.gmax = _gmax;
+  w.bmax = _bmax;
   const size_t bytes = gg_workspace_bytes(w.capacity, w.boundary_capacity, g, w.max_sweeps,
                                           w.gmax, w.bmax);
   if (bytes == 0) throw std::runtime_error("invalid workspace request");

@@ -476,6 +476,48 @@ int ggh_system_set_solver(ggh_system *h, int stab, int solver, float mu_s, float
   }
 }
 
+int ggh_system_set_lists(ggh_system *h, int gmax, int bmax, float skin) {
+  try {
+    if (!h) throw std::invalid_argument("null handle");
+    h->sys->get_solver().set_neighbor_lists(gmax, bmax, skin);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_set_merge_rules(ggh_system *h, int depth_rules, float min_mass, float split_mass,
+                               float merge_dist, float merge_depth, float split_depth,
+                               float max_mass) {
+  try {
+    if (!h) throw std::invalid_argument("null handle");
+    h->sys->set_merge_rules(depth_rules, min_mass, split_mass, merge_dist, merge_depth,
+                            split_depth, max_mass);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
+int ggh_system_set_learned(ggh_system *h, const unsigned short *w_bf16, const float *bias,
+                           float radius, float vscale, float guard_boundary, float guard_floor,
+                           float guard_side) {
+  try {
+    if (!h) throw std::invalid_argument("null handle");
+    gg_mlp_config c;
+    gg_mlp_config_default(&c);
+    c.radius = radius;
+    c.vscale = vscale;
+    c.guard_boundary = guard_boundary;
+    c.guard_floor = guard_floor;
+    c.guard_side = guard_side;
+    h->sys->set_learned_upsampler(reinterpret_cast<const uint16_t *>(w_bf16), bias, c);
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
 int ggh_write_vtk(const char *path, const char *title, const float *xyz, int n) {
   try {
     if (!path || !title || n < 0 || (n > 0 && !xyz)) throw std::invalid_argument("bad arguments");

@@ -5,7 +5,8 @@
 //
 // step(): boundary re-sort when the boundary moves, cell sort + PBD step of
 // the granular set, fine-particle sort + plain upsampler, then (adaptive)
-// surface flags, distance to the surface and merge/split.  Returns 1 and
+// surface flags, distance to the surface and merge/split.  With the learned
+// upsampler set, the fine set is rewritten by the network instead.  Returns 1 and
 // appends the frame time in ms to get_frame_times(), printing it as the
 // reference does (set_verbose(false) silences the print).  Exceptions are
 // caught and printed, as in the reference.
@@ -50,7 +51,7 @@ class GranularSystem {
     return static_cast<const std::shared_ptr<GranularParticles>>(_upsampled);
   }
 
-  float get_max_mass() const { return (float)_max_mass; }
+  float get_max_mass() const { return _merge_max_mass; }
   float get_min_mass() const { return (float)_min_mass; }
   float get_upsampled_radius() const { return _upsampled_radius; }
 
@@ -61,6 +62,21 @@ class GranularSystem {
   Solver &get_solver() { return _solver; }
 
   void set_verbose(bool v) { _verbose = v; }
+
+  // Merge/split rules of the adaptive passes (gg_params' adaptive_rules,
+  // min_mass .. split_depth) and the largest mass a merge may start from;
+  // the reference rules with max mass 4 until set.
+  void set_merge_rules(int depth_rules, float min_mass, float split_mass, float merge_dist,
+                       float merge_depth, float split_depth, float max_mass);
+
+  // Learned output: from the next step on, the fine set holds 8 particles per
+  // granular particle (fine index 8 i + k, slot k of the 2x2x2 template of
+  // granular particle i) written by gg_upsample_mlp instead of the plain
+  // upsampler's update.  w_bf16 (GG_MLP_WEIGHTS bfloat16) and bias
+  // (GG_MLP_BIASES) are host arrays, copied to the device.
+  void set_learned_upsampler(const uint16_t *w_bf16, const float *bias,
+                             const gg_mlp_config &cfg);
+  bool learned_upsampler() const { return _learned; }
 
  private:
   std::shared_ptr<GranularParticles> _particles;
@@ -85,7 +101,15 @@ class GranularSystem {
   const bool _is_move_boundary;
   const int _is_adaptive;
   bool _verbose = true;
+  float _merge_max_mass = 4.0f;
+  bool _learned = false;
+  std::unique_ptr<DArray<unsigned int>> _mlp_w;  // bfloat16 pairs
+  std::unique_ptr<DArray<float>> _mlp_b;
+  std::unique_ptr<DArray<unsigned int>> _mlp_scratch;
+  gg_mlp_config _mlp_cfg{};
   Solver _solver;
+
+  void learned_update();
 
   void ensure_capacity();
   void neighbor_search_granular(const std::shared_ptr<GranularParticles> &particles,

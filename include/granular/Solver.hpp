@@ -136,6 +136,16 @@ class Solver {
   void set_iterations(int stab, int solver);
   void set_friction(float mu_s, float mu_k);
   void set_flags(int flags);
+  // Merge/split rules (gg_params adaptive_rules, min_mass, split_mass,
+  // merge_dist, merge_depth, split_depth); the reference rules until set.
+  void set_merge_rules(int depth_rules, float min_mass, float split_mass, float merge_dist,
+                       float merge_depth, float split_depth);
+  // Neighbour lists: granular and boundary candidates per particle and the
+  // skin (m).  Defaults 48, 16, 0.0075 fit r = 0.01 with 0.01 boundary
+  // particles; smaller particles near dense boundaries need more boundary
+  // slots (a particle whose list overflows scans its whole stencil every
+  // sweep, with the same results).
+  void set_neighbor_lists(int gmax, int bmax, float skin);
 
  private:
   gg_params params(float dt, float3 G, float density) const;
@@ -160,6 +170,11 @@ class Solver {
   // boundary is rebuilt from them when the workspace is reallocated)
   std::shared_ptr<GranularParticles> _bnd;
   const DArray<int> *_bcs = nullptr;
+  int _gmax = 48, _bmax = 16;
+  float _skin = 0.0075f;
+  int _rules = 0;
+  float _min_mass = 1.0f, _split_mass = 2.0f, _merge_dist = 0.035f, _merge_depth = 3.0f,
+        _split_depth = 2.0f;
   int _stab = 3;
   int _solver_iters = 5;
   float _mu_s = 0.0f, _mu_k = 0.0f;

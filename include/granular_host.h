@@ -159,6 +159,22 @@ int ggh_system_save_checkpoint(const ggh_system *h, const char *path);
 int ggh_checkpoint_read(const char *path, int which, int field, void *out, int max);
 int ggh_system_set_solver(ggh_system *h, int stab, int solver, float mu_s, float mu_k,
                           int flags);
+/* Solver::set_neighbor_lists: granular and boundary candidates per
+ * neighbour list and the list skin (m); defaults 48, 16, 0.0075. */
+int ggh_system_set_lists(ggh_system *h, int gmax, int bmax, float skin);
+/* GranularSystem::set_merge_rules: merge/split rules of the adaptive passes
+ * (0 reference rules, 1 depth rules; the fields of gg_params) and the
+ * largest mass a merge may start from. */
+int ggh_system_set_merge_rules(ggh_system *h, int depth_rules, float min_mass, float split_mass,
+                               float merge_dist, float merge_depth, float split_depth,
+                               float max_mass);
+/* GranularSystem::set_learned_upsampler: from the next step on the fine set
+ * holds 8 particles per granular particle written by gg_upsample_mlp.
+ * w_bf16: GG_MLP_WEIGHTS bfloat16 bit patterns, bias: GG_MLP_BIASES floats
+ * (host arrays); the rest are gg_mlp_config's fields. */
+int ggh_system_set_learned(ggh_system *h, const unsigned short *w_bf16, const float *bias,
+                           float radius, float vscale, float guard_boundary, float guard_floor,
+                           float guard_side);
 
 /* Device self-test of DArray (compact, append, remove, resize,
  * verifyArrayOrdering) and GranularParticles (remove_elements,
