@@ -95,6 +95,16 @@ class gg_batch_scene(C.Structure):
                 ("bcell_start", C.c_void_p), ("params", gg_params), ("ws", gg_workspace)]
 
 
+class gg_scene_desc(C.Structure):
+    _fields_ = [("cx", C.c_int), ("cy", C.c_int), ("cz", C.c_int), ("cell_base", C.c_int),
+                ("mu_s", C.c_float), ("mu_k", C.c_float)]
+
+
+class gg_scenes(C.Structure):
+    _fields_ = [("count", C.c_int), ("desc", C.c_void_p), ("host_desc", C.c_void_p),
+                ("scene_of", C.c_void_p), ("bscene_of", C.c_void_p), ("friction", C.c_int)]
+
+
 class gg_batch_config(C.Structure):
     _fields_ = [("branches", C.c_int), ("sweep_block", C.c_int), ("nlist_block", C.c_int),
                 ("steps_per_launch", C.c_int)]
@@ -148,6 +158,17 @@ _PROTOS = {
     "gg_batch_set_config": (C.c_int, [VP, P(gg_batch_config)]),
     "gg_batch_autotune": (C.c_int, [VP, C.c_int, VP, P(gg_batch_config), P(C.c_float)]),
     "gg_batch_destroy": (None, [VP]),
+    "gg_scenes_boundary_build": (C.c_int, [P(gg_particles), VP, P(gg_scenes), P(gg_workspace),
+                                           VP]),
+    "gg_step_scenes": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, P(gg_scenes),
+                                 P(gg_params), P(gg_workspace), VP]),
+    "gg_scenes_graph_create": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP,
+                                         P(gg_scenes), P(gg_params), P(gg_workspace), C.c_int,
+                                         P(VP)]),
+    "gg_scenes_graph_launch": (C.c_int, [VP, VP]),
+    "gg_scenes_graph_autotune": (C.c_int, [VP, C.c_int, VP, P(C.c_int), P(C.c_int),
+                                           P(C.c_float)]),
+    "gg_scenes_graph_destroy": (None, [VP]),
     "gg_step_graph_create": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, P(gg_params),
                                        P(gg_workspace), C.c_int, P(VP)]),
     "gg_step_graph_launch": (C.c_int, [VP, VP]),

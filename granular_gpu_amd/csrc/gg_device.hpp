@@ -129,6 +129,28 @@ struct SweepConst {
   int gmax, bmax, cap;
 };
 
+// Scenes sharing one set (gg_step_scenes): the scene of each particle and
+// the scenes' grids, cell offsets and friction.  With SCENES false (one
+// scene) the launch's own grid at cell offset 0 and its own friction.
+struct SceneView {
+  const int *__restrict__ scene_of;
+  const gg_scene_desc *__restrict__ desc;
+  int count;  // scene indices are clamped to [0, count)
+};
+struct LocalGrid {
+  DevGrid g;
+  int base;  // offset of the scene's cells in the cell tables
+  float mu_s, mu_k;
+};
+template <bool SCENES>
+__device__ __forceinline__ LocalGrid local_grid(const SceneView &sv, int i, const DevGrid &g0,
+                                                const SweepConst &sc) {
+  if (!SCENES) return LocalGrid{g0, 0, sc.mu_s, sc.mu_k};
+  const int k = sv.scene_of[i];
+  const gg_scene_desc d = sv.desc[k < 0 ? 0 : (k < sv.count ? k : sv.count - 1)];
+  return LocalGrid{DevGrid{g0.L, d.cx, d.cy, d.cz}, d.cell_base, d.mu_s, d.mu_k};
+}
+
 // Correctly rounded square root, reciprocal and quotient on restricted
 // domains.  The compiler's IEEE sequences (what sqrtf and '/' compile to
 // under -fhip-fp32-correctly-rounded-divide-sqrt) wrap a core in steps that

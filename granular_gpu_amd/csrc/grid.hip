@@ -378,7 +378,8 @@ int gather_inplace(void *arr, int words, const int *perm, int n, void *tmp, hipS
 }
 
 int grid_build_general(gg_particles *set, gg_adaptive *ad, int *cell_start, int *perm_out,
-                       const gg_workspace *ws, hipStream_t st, int capacity_limit) {
+                       const gg_workspace *ws, hipStream_t st, int capacity_limit,
+                       const int *keys) {
   if (!set || !cell_start) return GG_ERR_INVALID;
   GG_TRY(check_ws(ws));
   const int n = set->n;
@@ -387,7 +388,8 @@ int grid_build_general(gg_particles *set, gg_adaptive *ad, int *cell_start, int 
   if (n > 0 && !set->pos) return GG_ERR_INVALID;
   Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
                          ws->bmax);
-  GG_TRY(sort_by_cell(reinterpret_cast<const float *>(set->pos), n, cell_start, ws, st));
+  GG_TRY(sort_by_cell_keys(reinterpret_cast<const float *>(set->pos), keys, n, cell_start, ws,
+                           st));
   const int *perm = at<int>(ws, L.perm);
   const int *key = at<int>(ws, L.key);
   if (n == 0) return GG_OK;

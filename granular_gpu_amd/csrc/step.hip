@@ -132,15 +132,21 @@ __global__ void __launch_bounds__(1024) k_reduce_bounds(const unsigned int *__re
 // |x*_i - x*_j| < r_i + r_j + skin (boundary: |x*_i - q| < R_b + max(r_i,
 // rmin) + skin), in scan order (cells ascending, index ascending).  x* is
 // the predicted position fl(base + offset) with the contact radius in w.
-template <int BS>
+// SCENES: the particle's scene gives the grid and the range of the cell
+// tables (meta keeps the scene-local stencil id and sorted cell).
+template <int BS, bool SCENES>
 __global__ void __launch_bounds__(BS)
     k_nlist(int n, const float4 *__restrict__ x, const float4 *__restrict__ xs,
-            const float4 *__restrict__ bnd4, const int *__restrict__ cs,
-            const int *__restrict__ bcs, DevGrid g, SweepConst sc, float skin,
+            const float4 *__restrict__ bnd4, const int *__restrict__ cs_all,
+            const int *__restrict__ bcs_all, DevGrid g0, SweepConst sc, float skin,
             int4 *__restrict__ meta, int *__restrict__ glist, int *__restrict__ blist,
-            int remap) {
+            int remap, SceneView sv) {
   const int i = xcd_block(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const LocalGrid lg = local_grid<SCENES>(sv, i, g0, sc);
+  const DevGrid &g = lg.g;
+  const int *__restrict__ cs = cs_all + lg.base;
+  const int *__restrict__ bcs = bcs_all + lg.base;
   const float4 si = xs[i];
   int bx = 0, by = 0, bz = 0;
   const bool ok = trunc_cell(si.x, si.y, si.z, g, bx, by, bz);
@@ -220,6 +226,7 @@ struct SweepArgs {
   int fb_wave_max;  // a wave hands at most this many fallbacks to whole waves
   const unsigned int *__restrict__ bounds;  // largest |D| component, contact radius
   float near_limit;  // a list stays usable one cell over while its reach is below this
+  SceneView sv;      // scenes sharing the set (read when SCENES)
 };
 
 // Shared epilogue: new offset (and for the last stabilisation sweep the
@@ -230,7 +237,7 @@ struct SweepArgs {
 template <bool LAST_STAB>
 __device__ __forceinline__ void sweep_store(const SweepArgs &a, int i, const float4 &bi,
                                             const float4 &oi, float3 ab, float3 ag, int cnt,
-                                            int cell) {
+                                            int cell, const LocalGrid &lg) {
   const float nf = (float)(cnt > 1 ? cnt : 1);
   const float ix = (ab.x + ag.x) / nf;
   const float iy = (ab.y + ag.y) / nf;
@@ -238,14 +245,15 @@ __device__ __forceinline__ void sweep_store(const SweepArgs &a, int i, const flo
   const float ox = oi.x + ix, oy = oi.y + iy, oz = oi.z + iz;
   const float cum_new = oi.w + sqrtf(ix * ix + iy * iy + iz * iz);
   if (a.use_lists && !(cum_new < a.sc.half_valid) && cum_new != oi.w) {
-    const DevGrid &g = a.g;
+    const DevGrid &g = lg.g;
     if (cell >= 0 && cell < g.cx * g.cy * g.cz) {
       const int cz0 = cell % g.cz, cy0 = (cell / g.cz) % g.cy, cx0 = cell / (g.cz * g.cy);
       // non-negative floats (and NaN above them) order like their bits
       const unsigned int mark = __float_as_uint(cum_new);
+      unsigned int *marks = reinterpret_cast<unsigned int *>(a.dirty) + lg.base;
       for (int m = 0; m < 27; ++m) {
         const int cc = stencil_cell(cx0, cy0, cz0, m, g);
-        if (cc >= 0) atomicMax(reinterpret_cast<unsigned int *>(a.dirty) + cc, mark);
+        if (cc >= 0) atomicMax(marks + cc, mark);
       }
     }
   }
@@ -260,15 +268,15 @@ __device__ __forceinline__ void sweep_store(const SweepArgs &a, int i, const flo
 
 // Full 27-cell stencil evaluation for particle i (row order = scan order).
 template <bool FRIC>
-__device__ __forceinline__ void full_stencil(const SweepArgs &a, int i, const float4 &bi,
+__device__ __forceinline__ void full_stencil(const SweepArgs &a, const SweepConst &sc,
+                                             const LocalGrid &lg, int i, const float4 &bi,
                                              const float4 &oi, int bx, int by, int bz, float rcb,
                                              float preb, float3 &ab, float3 &ag, int &nb,
                                              int &ng) {
-  const SweepConst &sc = a.sc;
-  stencil_rows(bx, by, bz, a.g, a.bcs, [&](int k0, int k1) {
+  stencil_rows(bx, by, bz, lg.g, a.bcs + lg.base, [&](int k0, int k1) {
     for (int k = k0; k < k1; ++k) boundary_contact<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, ab, nb);
   });
-  stencil_rows(bx, by, bz, a.g, a.cs, [&](int j0, int j1) {
+  stencil_rows(bx, by, bz, lg.g, a.cs + lg.base, [&](int j0, int j1) {
     int j = j0;
     for (; j + 4 <= j1; j += 4) {
       const float4 b0 = a.base[j], b1 = a.base[j + 1], b2 = a.base[j + 2], b3 = a.base[j + 3];
@@ -361,14 +369,14 @@ __device__ __forceinline__ void add_lanes(unsigned long long mask, const float3 
 // full_stencil's bit for bit.  Every lane of the wave calls it with the
 // same particle; the accumulators come back wave-uniform.
 template <bool FRIC>
-__device__ __forceinline__ void wave_stencil(const SweepArgs &a, int i, const float4 &bi,
+__device__ __forceinline__ void wave_stencil(const SweepArgs &a, const SweepConst &sc,
+                                             const LocalGrid &lg, int i, const float4 &bi,
                                              const float4 &oi, int bx, int by, int bz, float rcb,
                                              float preb, float3 &ab, float3 &ag, int &nb,
                                              int &ng) {
-  const SweepConst &sc = a.sc;
   const int lane = threadIdx.x & 63;
   StencilRows rb, rg;
-  stencil_row_tables(bx, by, bz, a.g, a.bcs, a.cs, lane, rb, rg);
+  stencil_row_tables(bx, by, bz, lg.g, a.bcs + lg.base, a.cs + lg.base, lane, rb, rg);
   // The first 2 x 64 boundary and 4 x 64 granular candidates are loaded
   // before any is used (one memory latency instead of one per 64).
   const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -413,14 +421,14 @@ __device__ __forceinline__ void wave_stencil(const SweepArgs &a, int i, const fl
 // loaded when it is used): the variant k_sweep inlines, where registers set
 // the occupancy of the list work.
 template <bool FRIC>
-__device__ __forceinline__ void wave_stencil_seq(const SweepArgs &a, int i, const float4 &bi,
+__device__ __forceinline__ void wave_stencil_seq(const SweepArgs &a, const SweepConst &sc,
+                                                 const LocalGrid &lg, int i, const float4 &bi,
                                                  const float4 &oi, int bx, int by, int bz,
                                                  float rcb, float preb, float3 &ab, float3 &ag,
                                                  int &nb, int &ng) {
-  const SweepConst &sc = a.sc;
   const int lane = threadIdx.x & 63;
   StencilRows rb, rg;
-  stencil_row_tables(bx, by, bz, a.g, a.bcs, a.cs, lane, rb, rg);
+  stencil_row_tables(bx, by, bz, lg.g, a.bcs + lg.base, a.cs + lg.base, lane, rb, rg);
   for (int t0 = 0; t0 < rb.total; t0 += 64) {
     const int k = stencil_row_index(rb, t0 + lane);
     float3 c = make_float3(0.0f, 0.0f, 0.0f), f = c;
@@ -447,7 +455,7 @@ __device__ __forceinline__ void wave_stencil_seq(const SweepArgs &a, int i, cons
 // or (when few lanes of the wave need it) queued for k_sweep_queue, which
 // scans each with a whole wave.  All paths visit contacts in
 // the same order, so the result does not depend on which path ran.
-template <bool FRIC, bool LAST_STAB, int BS>
+template <bool FRIC, bool LAST_STAB, int BS, bool SCENES>
 __global__ void __launch_bounds__(BS, 6) k_sweep(SweepArgs a) {
   __shared__ int lqueue[BS];
   __shared__ int lqueued;
@@ -458,8 +466,13 @@ __global__ void __launch_bounds__(BS, 6) k_sweep(SweepArgs a) {
     __syncthreads();
   }
   const int i = xcd_block(blockIdx.x, gridDim.x, a.remap) * blockDim.x + threadIdx.x;
-  const DevGrid &g = a.g;
-  const SweepConst &sc = a.sc;
+  // the particle's grid, cell-table offset and friction (its scene's)
+  LocalGrid lg{a.g, 0, a.sc.mu_s, a.sc.mu_k};
+  if (SCENES && i < a.n) lg = local_grid<SCENES>(a.sv, i, a.g, a.sc);
+  const DevGrid &g = lg.g;
+  SweepConst sc = a.sc;
+  sc.mu_s = lg.mu_s;
+  sc.mu_k = lg.mu_k;
   // 0 idle lane, 1 list, 2 stencil scan, 3 stencil outside the grid
   int mode = 0;
   float4 bi = make_float4(0.0f, 0.0f, 0.0f, 0.0f), oi = bi;
@@ -490,7 +503,8 @@ __global__ void __launch_bounds__(BS, 6) k_sweep(SweepArgs a) {
       // unless j marked the cell.
       bool moved_ok = false;
       if (bcell >= 0 && oi.w < sc.half_valid) {
-        const unsigned int mark = reinterpret_cast<const unsigned int *>(a.dirty)[bcell];
+        const unsigned int mark =
+            reinterpret_cast<const unsigned int *>(a.dirty)[lg.base + bcell];
         moved_ok = mark == 0u || oi.w + __uint_as_float(mark) < 2.0f * sc.half_valid;
       }
       list_ok = base_ok && moved_ok && mt.x <= sc.gmax && mt.y <= sc.bmax;
@@ -559,26 +573,31 @@ __global__ void __launch_bounds__(BS, 6) k_sweep(SweepArgs a) {
       granular_contact<FRIC>(bi, oi, a.base[j], a.off_in[j], sc, ag, ng);
     }
   } else if (mode == 2 && !queued) {
-    full_stencil<FRIC>(a, i, bi, oi, bx, by, bz, rcb, preb, ab, ag, nb, ng);
+    full_stencil<FRIC>(a, sc, lg, i, bi, oi, bx, by, bz, rcb, preb, ab, ag, nb, ng);
   }
   if (mode == 1 || mode == 3 || (mode == 2 && !queued))
-    sweep_store<LAST_STAB>(a, i, bi, oi, ab, ag, nb + ng, mt.w);
+    sweep_store<LAST_STAB>(a, i, bi, oi, ab, ag, nb + ng, mt.w, lg);
   if (!local_q) return;
   // the block's queued particles: one wave each
   __syncthreads();
   const int nq = lqueued;
   for (int q = threadIdx.x >> 6; q < nq; q += BS / 64) {
     const int iq = lqueue[q];
+    const LocalGrid lq = local_grid<SCENES>(a.sv, iq, a.g, a.sc);
+    SweepConst scq = a.sc;
+    scq.mu_s = lq.mu_s;
+    scq.mu_k = lq.mu_k;
     const float4 bq = a.base[iq];
     const float4 oq = a.off_in[iq];
     int qx = 0, qy = 0, qz = 0;
-    trunc_cell(bq.x + oq.x, bq.y + oq.y, bq.z + oq.z, g, qx, qy, qz);
-    const float rq = sc.rb + fmaxf(bq.w, sc.rmin);
+    trunc_cell(bq.x + oq.x, bq.y + oq.y, bq.z + oq.z, lq.g, qx, qy, qz);
+    const float rq = scq.rb + fmaxf(bq.w, scq.rmin);
     float3 qb = make_float3(0.0f, 0.0f, 0.0f), qg = qb;
     int nqb = 0, nqg = 0;
-    wave_stencil_seq<FRIC>(a, iq, bq, oq, qx, qy, qz, rq, contact_prefilter(rq), qb, qg, nqb,
-                           nqg);
-    if (lane == 0) sweep_store<LAST_STAB>(a, iq, bq, oq, qb, qg, nqb + nqg, a.meta[iq].w);
+    wave_stencil_seq<FRIC>(a, scq, lq, iq, bq, oq, qx, qy, qz, rq, contact_prefilter(rq), qb,
+                           qg, nqb, nqg);
+    if (lane == 0)
+      sweep_store<LAST_STAB>(a, iq, bq, oq, qb, qg, nqb + nqg, a.meta[iq].w, lq);
   }
 }
 
@@ -591,28 +610,32 @@ __global__ void __launch_bounds__(BS, 6) k_sweep(SweepArgs a) {
 // step).  A queue that was not cleared (a sweep repeated without its
 // neighbours) only makes particles scan again: the full-stencil result is
 // every particle's exact result, so rewriting it changes nothing.
-template <bool FRIC, bool LAST_STAB>
+template <bool FRIC, bool LAST_STAB, bool SCENES>
 __global__ void __launch_bounds__(256) k_sweep_queue(SweepArgs a) {
   const int lane = threadIdx.x & 63;
-  const DevGrid &g = a.g;
-  const SweepConst &sc = a.sc;
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.fbq_next = 0;
   const int len = __atomic_load_n(a.fbq_count, __ATOMIC_RELAXED);
-  const int nq = len < sc.cap ? len : sc.cap;
+  const int nq = len < a.sc.cap ? len : a.sc.cap;
   const int waves = gridDim.x * (blockDim.x >> 6);
   for (int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); q < nq; q += waves) {
     const int iq = a.fbq[q];
     if (iq < 0 || iq >= a.n) continue;
+    const LocalGrid lq = local_grid<SCENES>(a.sv, iq, a.g, a.sc);
+    SweepConst sc = a.sc;
+    sc.mu_s = lq.mu_s;
+    sc.mu_k = lq.mu_k;
     const float4 bq = a.base[iq];
     const float4 oq = a.off_in[iq];
     int qx = 0, qy = 0, qz = 0;
     // (only a stale entry can fail this; k_sweep stored its result)
-    if (!trunc_cell(bq.x + oq.x, bq.y + oq.y, bq.z + oq.z, g, qx, qy, qz)) continue;
+    if (!trunc_cell(bq.x + oq.x, bq.y + oq.y, bq.z + oq.z, lq.g, qx, qy, qz)) continue;
     const float rq = sc.rb + fmaxf(bq.w, sc.rmin);
     float3 qb = make_float3(0.0f, 0.0f, 0.0f), qg = qb;
     int nqb = 0, nqg = 0;
-    wave_stencil<FRIC>(a, iq, bq, oq, qx, qy, qz, rq, contact_prefilter(rq), qb, qg, nqb, nqg);
-    if (lane == 0) sweep_store<LAST_STAB>(a, iq, bq, oq, qb, qg, nqb + nqg, a.meta[iq].w);
+    wave_stencil<FRIC>(a, sc, lq, iq, bq, oq, qx, qy, qz, rq, contact_prefilter(rq), qb, qg, nqb,
+                       nqg);
+    if (lane == 0)
+      sweep_store<LAST_STAB>(a, iq, bq, oq, qb, qg, nqb + nqg, a.meta[iq].w, lq);
   }
 }
 
@@ -690,31 +713,46 @@ static SweepConst sweep_const(const gg_params *p, const gg_workspace *ws) {
   return sc;
 }
 
-template <int BS>
+template <int BS, bool SCENES>
 static int launch_nlist_bs(int n, const float4 *x, const float4 *xs, const float4 *bnd4,
                            const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
-                           int4 *meta, int *glist, int *blist, int remap, hipStream_t st) {
-  k_nlist<BS><<<(n + BS - 1) / BS, BS, 0, st>>>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta,
-                                                 glist, blist, remap);
+                           int4 *meta, int *glist, int *blist, int remap, SceneView sv,
+                           hipStream_t st) {
+  k_nlist<BS, SCENES><<<(n + BS - 1) / BS, BS, 0, st>>>(n, x, xs, bnd4, cs, bcs, g, sc, skin,
+                                                         meta, glist, blist, remap, sv);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
 
-static int launch_nlist(int bs, int n, const float4 *x, const float4 *xs, const float4 *bnd4,
-                        const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
-                        int4 *meta, int *glist, int *blist, int remap, hipStream_t st) {
+template <bool SCENES>
+static int launch_nlist_sc(int bs, int n, const float4 *x, const float4 *xs, const float4 *bnd4,
+                           const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
+                           int4 *meta, int *glist, int *blist, int remap, SceneView sv,
+                           hipStream_t st) {
   switch (bs) {
-    case 64: return launch_nlist_bs<64>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
-    case 128: return launch_nlist_bs<128>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
-    case 512: return launch_nlist_bs<512>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
-    default: return launch_nlist_bs<256>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, st);
+    case 64: return launch_nlist_bs<64, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, st);
+    case 128: return launch_nlist_bs<128, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, st);
+    case 512: return launch_nlist_bs<512, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, st);
+    default: return launch_nlist_bs<256, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, st);
   }
 }
 
-// permute (perm may be null = identity) + predict + lists.
+static int launch_nlist(int bs, int n, const float4 *x, const float4 *xs, const float4 *bnd4,
+                        const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
+                        int4 *meta, int *glist, int *blist, int remap, const SceneView *sv,
+                        hipStream_t st) {
+  if (sv)
+    return launch_nlist_sc<true>(bs, n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist,
+                                 remap, *sv, st);
+  return launch_nlist_sc<false>(bs, n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist,
+                                remap, SceneView{nullptr, nullptr, 0}, st);
+}
+
+// permute (perm may be null = identity) + predict + lists.  sv: scenes
+// sharing the set (null: one scene).
 static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, const int *key,
                        const int *cs, const int *bcs, int nbnd, const gg_params *p,
-                       const gg_workspace *ws, hipStream_t st) {
+                       const gg_workspace *ws, hipStream_t st, const SceneView *sv = nullptr) {
   Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
                          ws->bmax);
   const int n = set->n;
@@ -742,44 +780,58 @@ static int run_predict(gg_particles *set, gg_adaptive *ad, const int *perm, cons
     GG_TRY(launch_nlist(p->nlist_block == 0 ? 64 : p->nlist_block, n, at<float4>(ws, L.x0), at<float4>(ws, L.off1),
                         at<float4>(ws, L.bnd4), cs, bcs, dev_grid(ws->grid), sweep_const(p, ws),
                         p->skin, at<int4>(ws, L.meta), at<int>(ws, L.glist),
-                        at<int>(ws, L.blist), (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1, st));
+                        at<int>(ws, L.blist), (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1, sv,
+                        st));
     prof_end(GG_STAGE_NLIST, st);
   }
   (void)nbnd;
   return GG_OK;
 }
 
-template <bool FRIC, bool LAST>
-static int launch_sweep(const SweepArgs &a, int bs, hipStream_t st) {
+template <bool FRIC, bool LAST, bool SCENES>
+static int launch_sweep_sc(const SweepArgs &a, int bs, hipStream_t st) {
   const int n = a.n;
   switch (bs) {
-    case 64: k_sweep<FRIC, LAST, 64><<<(n + 63) / 64, 64, 0, st>>>(a); break;
-    case 128: k_sweep<FRIC, LAST, 128><<<(n + 127) / 128, 128, 0, st>>>(a); break;
-    case 512: k_sweep<FRIC, LAST, 512><<<(n + 511) / 512, 512, 0, st>>>(a); break;
-    default: k_sweep<FRIC, LAST, 256><<<(n + 255) / 256, 256, 0, st>>>(a); break;
+    case 64: k_sweep<FRIC, LAST, 64, SCENES><<<(n + 63) / 64, 64, 0, st>>>(a); break;
+    case 128: k_sweep<FRIC, LAST, 128, SCENES><<<(n + 127) / 128, 128, 0, st>>>(a); break;
+    case 512: k_sweep<FRIC, LAST, 512, SCENES><<<(n + 511) / 512, 512, 0, st>>>(a); break;
+    default: k_sweep<FRIC, LAST, 256, SCENES><<<(n + 255) / 256, 256, 0, st>>>(a); break;
   }
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
 
 template <bool FRIC, bool LAST>
+static int launch_sweep(const SweepArgs &a, int bs, hipStream_t st) {
+  return a.sv.scene_of ? launch_sweep_sc<FRIC, LAST, true>(a, bs, st)
+                       : launch_sweep_sc<FRIC, LAST, false>(a, bs, st);
+}
+
+template <bool FRIC, bool LAST>
 static int launch_sweep_queue(const SweepArgs &a, hipStream_t st) {
   // up to 2048 waves take the queued particles in turn
   const int qblocks = a.n / 256 + 1 < 512 ? a.n / 256 + 1 : 512;
-  k_sweep_queue<FRIC, LAST><<<qblocks, 256, 0, st>>>(a);
+  if (a.sv.scene_of)
+    k_sweep_queue<FRIC, LAST, true><<<qblocks, 256, 0, st>>>(a);
+  else
+    k_sweep_queue<FRIC, LAST, false><<<qblocks, 256, 0, st>>>(a);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
 
+// fric_scenes: with scenes, whether their sweeps apply friction (each with
+// its own coefficients).
 static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, const gg_params *p,
-                     int k, const gg_workspace *ws, hipStream_t st) {
+                     int k, const gg_workspace *ws, hipStream_t st,
+                     const SceneView *sv = nullptr, bool fric_scenes = false) {
   Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
                          ws->bmax);
   const int n = set->n;
   if (n <= 0) return GG_OK;
   const int S = p->stab_iters;
   const bool last_stab = (k == S - 1);
-  const bool fric = (k >= S) && (p->mu_s > 0.0f || p->mu_k > 0.0f);
+  const bool fric =
+      (k >= S) && (sv ? fric_scenes : (p->mu_s > 0.0f || p->mu_k > 0.0f));
   SweepArgs a;
   a.n = n;
   a.base = (k < S || S == 0) ? at<float4>(ws, L.x0) : at<float4>(ws, L.xb);
@@ -804,6 +856,7 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   a.remap = (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1;
   a.bounds = at<unsigned int>(ws, L.bounds);
   a.near_limit = 0.999f * ws->grid.cell_length;
+  a.sv = sv ? *sv : SceneView{nullptr, nullptr, 0};
   // GG_FLAG_FALLBACK_WAVE queues every stencil scan sweep-wide, whatever the
   // size of the set
   a.fbq_global = (p->flags & GG_FLAG_FALLBACK_WAVE) ? 1 : 0;
@@ -901,6 +954,36 @@ extern "C" int gg_step(gg_particles *set, gg_adaptive *ad, int *cell_start,
   for (int k = 0; k < K; ++k)
     GG_TRY(run_sweep(set, cell_start, bcell_start, params, k, ws, stream));
   return run_finalize(set, ad, params, ws, stream);
+}
+
+extern "C" int gg_step_scenes(gg_particles *set, int *cell_start, const gg_particles *boundary,
+                              const int *bcell_start, const gg_scenes *scenes,
+                              const gg_params *params, const gg_workspace *ws,
+                              hipStream_t stream) {
+  GG_TRY(check_ws(ws));
+  GG_TRY(check_params(params, ws));
+  GG_TRY(check_set(set, ws));
+  GG_TRY(check_boundary(boundary, bcell_start, ws));
+  if (!cell_start) return GG_ERR_INVALID;
+  GG_TRY(check_scenes(scenes, ws));
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  const SceneView sv{scenes->scene_of, scenes->desc, scenes->count};
+  // cell keys over the concatenated tables, then the ordinary sort
+  int *keys = at<int>(ws, L.itmp2);
+  prof_begin(GG_STAGE_SORT, stream);
+  GG_TRY(scene_keys(reinterpret_cast<const float *>(set->pos), set->n, scenes->scene_of, scenes,
+                    ws->grid.cell_length, keys, stream));
+  GG_TRY(sort_by_cell_keys(reinterpret_cast<const float *>(set->pos), keys, set->n, cell_start,
+                           ws, stream));
+  prof_end(GG_STAGE_SORT, stream);
+  GG_TRY(run_predict(set, nullptr, at<int>(ws, L.perm), at<int>(ws, L.key), cell_start,
+                     bcell_start, boundary->n, params, ws, stream, &sv));
+  const int K = params->stab_iters + params->solver_iters;
+  for (int k = 0; k < K; ++k)
+    GG_TRY(run_sweep(set, cell_start, bcell_start, params, k, ws, stream, &sv,
+                     scenes->friction != 0));
+  return run_finalize(set, nullptr, params, ws, stream);
 }
 
 extern "C" int gg_predict(gg_particles *set, const int *cell_start, const gg_particles *boundary,

@@ -347,3 +347,145 @@ class BatchSim:
             self.close()
         except Exception:
             pass
+
+
+class SceneBatch:
+    """Independent plain scenes stored back to back in one set and advanced
+    by one launch per stage (gg_step_scenes; captured: gg_scenes_graph_*).
+    `scenes` are scenes.Scene objects sharing cell length, dt, gravity and
+    density; `friction` is a list of (mu_s, mu_k) per scene.  Scene s's
+    particles are the rows first[s]:first[s + 1] of the set (cell-sorted
+    within the scene after every step)."""
+
+    def __init__(self, scenes, friction, stab=3, solver=5, skin=0.0075, gmax=48, bmax=16,
+                 flags=0, device="cuda"):
+        lib = load()
+        self.lib = lib
+        self.scenes = list(scenes)
+        s0 = self.scenes[0]
+        for sc in self.scenes:
+            if (np.float32(sc.cell_length) != np.float32(s0.cell_length)
+                    or np.float32(sc.dt) != np.float32(s0.dt)
+                    or tuple(sc.gravity) != tuple(s0.gravity)
+                    or np.float32(sc.density) != np.float32(s0.density)):
+                raise ValueError("SceneBatch: scenes must share cell length, dt, gravity and "
+                                 "density")
+        if len(friction) != len(self.scenes):
+            raise ValueError("SceneBatch: one (mu_s, mu_k) per scene")
+        fr = [(float(a), float(b)) for a, b in friction]
+        with_f = [a > 0.0 or b > 0.0 for a, b in fr]
+        if any(with_f) and not all(with_f):
+            raise ValueError("SceneBatch: friction on every scene or on none")
+        dev = torch.device(device)
+        self.dev = dev
+        ns = [sc.n for sc in self.scenes]
+        nbs = [sc.boundary.shape[0] for sc in self.scenes]
+        self.first = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+        self.bfirst = np.concatenate([[0], np.cumsum(nbs)]).astype(np.int64)
+        n, nb = int(self.first[-1]), int(self.bfirst[-1])
+        self.n, self.nb = n, nb
+        desc = (_lib.gg_scene_desc * len(self.scenes))()
+        base = 0
+        for k, sc in enumerate(self.scenes):
+            cx, cy, cz = sc.cells
+            desc[k] = _lib.gg_scene_desc(cx, cy, cz, base, fr[k][0], fr[k][1])
+            base += cx * cy * cz + 1
+        self.total_cells = base
+        self.host_desc = desc
+        raw = np.frombuffer(bytes(desc), dtype=np.uint8)
+        self.desc_dev = torch.from_numpy(raw.copy()).to(dev)
+        f32, i32 = torch.float32, torch.int32
+        cat = lambda xs: np.ascontiguousarray(np.concatenate(xs), np.float32)
+        self.pos = torch.from_numpy(cat([sc.pos for sc in self.scenes])).to(dev)
+        self.vel = torch.from_numpy(cat([sc.vel for sc in self.scenes])).to(dev)
+        self.mass = torch.from_numpy(cat([sc.mass for sc in self.scenes])).to(dev)
+        self.scaled_mass = torch.zeros(n, dtype=f32, device=dev)
+        self.p2c = torch.zeros(n, dtype=i32, device=dev)
+        self.adl = torch.zeros(n, dtype=i32, device=dev)
+        self.scene_of = torch.from_numpy(
+            np.repeat(np.arange(len(self.scenes), dtype=np.int32), ns)).to(dev)
+        self.bpos = torch.from_numpy(cat([sc.boundary for sc in self.scenes])).to(dev)
+        self.bvel = torch.zeros_like(self.bpos)
+        self.bmass = torch.ones(max(nb, 1), dtype=f32, device=dev)
+        self.bp2c = torch.zeros(max(nb, 1), dtype=i32, device=dev)
+        self.banim = torch.zeros(max(nb, 1), dtype=i32, device=dev)
+        self.bscene_of = torch.from_numpy(
+            np.repeat(np.arange(len(self.scenes), dtype=np.int32), nbs)).to(dev)
+        self.cs = torch.zeros(base + 1, dtype=i32, device=dev)
+        self.bcs = torch.zeros(base + 1, dtype=i32, device=dev)
+        self.grid = _lib.gg_grid(float(s0.cell_length), base - 1, 1, 1)
+        p = _lib.default_params()
+        p.dt = float(s0.dt)
+        p.gx, p.gy, p.gz = (float(v) for v in s0.gravity)
+        p.density = float(s0.density)
+        p.stab_iters, p.solver_iters = stab, solver
+        p.skin = skin
+        p.flags = flags
+        self.params = p
+        nbytes = lib.gg_workspace_bytes(max(n, 1), max(nb, 1), self.grid, stab + solver, gmax,
+                                        bmax)
+        if nbytes == 0:
+            raise ValueError("invalid workspace request")
+        self.ws_buf = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        self.ws = _lib.gg_workspace(self.ws_buf.data_ptr(), int(nbytes), max(n, 1), max(nb, 1),
+                                    self.grid, stab + solver, gmax, bmax)
+        check("gg_workspace_init", lib.gg_workspace_init(C.byref(self.ws), _stream()))
+        self.set = _lib.gg_particles(n, n, _ptr(self.pos), _ptr(self.vel), _ptr(self.mass),
+                                     _ptr(self.scaled_mass), None, None, None, _ptr(self.p2c),
+                                     None, _ptr(self.adl))
+        self.bset = _lib.gg_particles(nb, max(nb, 1), _ptr(self.bpos), _ptr(self.bvel),
+                                      _ptr(self.bmass), None, None, None, None, _ptr(self.bp2c),
+                                      _ptr(self.banim), None)
+        self.sc = _lib.gg_scenes(len(self.scenes), self.desc_dev.data_ptr(), C.addressof(desc),
+                                 self.scene_of.data_ptr(), self.bscene_of.data_ptr(),
+                                 1 if any(with_f) else 0)
+        check("gg_scenes_boundary_build",
+              lib.gg_scenes_boundary_build(C.byref(self.bset), _ptr(self.bcs), C.byref(self.sc),
+                                           C.byref(self.ws), _stream()))
+        self.graph = None
+
+    def step(self, nsteps=1):
+        for _ in range(nsteps):
+            check("gg_step_scenes",
+                  self.lib.gg_step_scenes(C.byref(self.set), _ptr(self.cs), C.byref(self.bset),
+                                          _ptr(self.bcs), C.byref(self.sc),
+                                          C.byref(self.params), C.byref(self.ws), _stream()))
+
+    def capture(self, steps_per_launch=1):
+        """The step captured into a hipGraph (gg_scenes_graph_create)."""
+        self.close_graph()
+        h = C.c_void_p()
+        check("gg_scenes_graph_create", self.lib.gg_scenes_graph_create(
+            C.byref(self.set), _ptr(self.cs), C.byref(self.bset), _ptr(self.bcs),
+            C.byref(self.sc), C.byref(self.params), C.byref(self.ws), steps_per_launch,
+            C.byref(h)))
+        self.graph = h
+
+    def launch(self, launches=1):
+        for _ in range(launches):
+            check("gg_scenes_graph_launch",
+                  self.lib.gg_scenes_graph_launch(self.graph, _stream()))
+
+    def autotune(self, trials=2):
+        sw, nl, ms = C.c_int(), C.c_int(), C.c_float()
+        check("gg_scenes_graph_autotune", self.lib.gg_scenes_graph_autotune(
+            self.graph, trials, _stream(), C.byref(sw), C.byref(nl), C.byref(ms)))
+        return dict(sweep_block=sw.value, nlist_block=nl.value, ms_per_launch=ms.value)
+
+    def scene(self, k):
+        """Host copies (pos, vel, mass) of scene k's particles."""
+        torch.cuda.synchronize()
+        a, b = int(self.first[k]), int(self.first[k + 1])
+        return (self.pos[a:b].cpu().numpy().copy(), self.vel[a:b].cpu().numpy().copy(),
+                self.mass[a:b].cpu().numpy().copy())
+
+    def close_graph(self):
+        if getattr(self, "graph", None):
+            self.lib.gg_scenes_graph_destroy(self.graph)
+            self.graph = None
+
+    def __del__(self):
+        try:
+            self.close_graph()
+        except Exception:
+            pass

@@ -471,6 +471,63 @@ int gg_bf16_round_check(uint64_t first, uint64_t count, unsigned long long *mism
 int gg_fast_math_check(int mode, uint64_t first, uint64_t count, unsigned long long *mismatches,
                        hipStream_t stream);
 
+/* ---- scenes sharing one set ------------------------------------------------ */
+/* Independent plain scenes stored back to back in one particle set and one
+ * boundary set, advanced by one launch per stage (no reference counterpart:
+ * the specification's batched configuration).  Scene s owns a contiguous
+ * range of the set and of the boundary, its own grid (cx, cy, cz; the cell
+ * length is the workspace's), and the cells [cell_base, cell_base + cx cy cz]
+ * of the concatenated cell tables: its grid cells, then its own sentinel
+ * (cell_base of scene s + 1 = cell_base of s + cx cy cz + 1).  A particle's
+ * cell id is its scene's cell_base plus its cell in its scene's grid, so
+ * the stable sort keeps every scene contiguous and in the order its own
+ * sort would give; every stencil is read through the scene's range of the
+ * tables, so each scene's result equals gg_step on that scene alone, bit
+ * for bit.  The workspace's grid is (L, T - 1, 1, 1) with T = the total of
+ * the scenes' cx cy cz + 1; its capacity covers the whole set. */
+typedef struct gg_scene_desc {
+  int cx, cy, cz;     /* the scene's grid                                   */
+  int cell_base;      /* its first cell in the concatenated cell tables     */
+  float mu_s, mu_k;   /* friction of its solver sweeps                      */
+} gg_scene_desc;
+typedef struct gg_scenes {
+  int count;
+  const gg_scene_desc *desc;       /* device array [count]                  */
+  const gg_scene_desc *host_desc;  /* the same on the host (the calls check
+                                      the cell ranges against it)           */
+  const int *scene_of;   /* device: scene of each granular particle, in
+                            scene order (non-decreasing)                    */
+  const int *bscene_of;  /* device: scene of each boundary particle, the
+                            same way                                        */
+  int friction;               /* nonzero: the solver sweeps apply friction
+                                 with each scene's mu_s, mu_k (scenes with
+                                 zero coefficients take the same arithmetic) */
+} gg_scenes;
+/* Sorts the boundary set (scenes back to back) into bcell_start and packs
+ * it into the workspace; gg_boundary_build for scenes. */
+int gg_scenes_boundary_build(gg_particles *boundary, int *bcell_start, const gg_scenes *scenes,
+                             const gg_workspace *ws, hipStream_t stream);
+/* One plain step of every scene: gg_step with per-scene grids and friction
+ * (params' own mu_s, mu_k are not read). */
+int gg_step_scenes(gg_particles *set, int *cell_start, const gg_particles *boundary,
+                   const int *bcell_start, const gg_scenes *scenes, const gg_params *params,
+                   const gg_workspace *ws, hipStream_t stream);
+/* gg_step_scenes captured into a hipGraph: steps_per_launch steps per
+ * launch, the list build and sweeps at the given block sizes (0 = 64);
+ * arguments are captured by value, device arrays by address. */
+typedef struct gg_scenes_graph gg_scenes_graph;
+int gg_scenes_graph_create(gg_particles *set, int *cell_start, const gg_particles *boundary,
+                           const int *bcell_start, const gg_scenes *scenes,
+                           const gg_params *params, const gg_workspace *ws, int steps_per_launch,
+                           gg_scenes_graph **out);
+int gg_scenes_graph_launch(gg_scenes_graph *g, hipStream_t stream);
+/* Times every sweep / list block size pair (64..512 / 128, 256) over
+ * `trials` launches, restoring pos, vel and mass after each, and recaptures
+ * with the fastest; best_ms (optional) = its time per launch. */
+int gg_scenes_graph_autotune(gg_scenes_graph *g, int trials, hipStream_t stream,
+                             int *sweep_block, int *nlist_block, float *best_ms);
+void gg_scenes_graph_destroy(gg_scenes_graph *g);
+
 /* ---- batched scenes -------------------------------------------------------- */
 /* Many independent plain scenes advanced together by one hipGraph launch
  * (no reference counterpart: the specification's batched-scenes

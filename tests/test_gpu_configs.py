@@ -80,6 +80,41 @@ def test_batched_scenes_each_match_oracle(gpu):
     b.close()
 
 
+@pytest.mark.parametrize("captured", [False, True])
+def test_scenes_in_one_set_each_match_oracle(gpu, captured):
+    # the same eight scenes stored back to back in one set: one launch per
+    # stage for all of them (eagerly, or replayed from a captured graph with
+    # eager work and synchronisations between the replays), each scene bit
+    # for bit against the oracle at its own friction
+    from granular_gpu_amd.device import SceneBatch
+    sizes = scenes.batch_box_sizes(64)
+    pick = _spread(sizes, 8)
+    dt, steps = 1.0 / 240.0, 10
+    scs = [scenes.batch_box(sizes[k][0], seed=1 + k, dt=dt, g=-9.81) for k in pick]
+    fr = [(sizes[k][1], sizes[k][2]) for k in pick]
+    b = SceneBatch(scs, fr, stab=3, solver=10)
+    _log(f"{len(scs)} scenes in one set: n = {b.n}, cells = {b.total_cells}, "
+         f"captured = {captured}")
+    if captured:
+        b.capture(steps_per_launch=1)
+        for _ in range(steps):
+            b.launch(1)
+            torch.cuda.synchronize()
+            torch.zeros(1024, device="cuda").sum().item()  # eager work between replays
+    else:
+        b.step(steps)
+    for k, (sc, (mu_s, mu_k)) in enumerate(zip(scs, fr)):
+        gp, gv, gm = b.scene(k)
+        p = po.params_from(sc, 3, 10, mu_s, mu_k)
+        bpos, bcs = po.sorted_boundary(sc.boundary, sc.cell_length, sc.cells)
+        op, ov, om = po.plain_steps_f32(sc.pos, sc.vel, sc.mass, bpos, bcs, p, steps)[:3]
+        _log(f"scene {k}: n={sc.n} mu_s={mu_s:.3f}: {int((gp != op).any(axis=1).sum())} differ")
+        assert np.array_equal(gp, op)
+        assert np.array_equal(gv, ov)
+        assert np.array_equal(gm, om)
+    b.close_graph()
+
+
 def _bf16_to_f32(t: torch.Tensor) -> np.ndarray:
     u = t.cpu().numpy().astype(np.uint16).astype(np.uint32) << 16
     return u.view(np.float32)
