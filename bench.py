@@ -317,52 +317,80 @@ def scale_phase(args, lib, torch, C, np, learned):
 def batch_phase(args, lib, torch, C, np):
     """Batched scenes: 64 independent plain box scenes with particle counts
     log-uniform in [4k, 256k] (seeded; 4.17M particles in all), per-scene
-    friction mu_s in [0.3, 0.7], mu_k = 0.8 mu_s, 3 + 10 sweeps, dt = 0.002,
-    advanced by one hipGraph launch per step after the autotuner has picked
-    branch count and block sizes.  The eager rate (same kernels launched
-    one by one, scene after scene) is reported beside it."""
+    friction mu_s in [0.3, 0.7], mu_k = 0.8 mu_s, 3 + 10 sweeps, dt = 1/240
+    (the plain box's), g = -9.81.  Headline: the scenes stored back to back
+    in one set and advanced by gg_step_scenes, one launch per stage for all
+    64, captured in a hipGraph whose block sizes the tuner picked.  Beside
+    it: the per-scene graph (every scene's own step chain on parallel
+    branches) and the same kernels launched eagerly scene after scene."""
     from granular_gpu_amd import scenes
-    from granular_gpu_amd.device import BatchSim, PlainSim
+    from granular_gpu_amd.device import BatchSim, PlainSim, SceneBatch
 
+    dt, g = args.batch_dt, -9.81
     sizes = scenes.batch_box_sizes(args.batch_scenes)
-    sims = [PlainSim(scenes.batch_box(side, seed=1 + k), stab=3, solver=10, mu_s=ms, mu_k=mk)
-            for k, (side, ms, mk) in enumerate(sizes)]
-    total = sum(s.n for s in sims)
-    bytes_step = sum(s.n * bytes_per_particle_step(s.n, s.C, s.nb, 3, 13) for s in sims)
-    print(f"# batch: {len(sims)} scenes, {total} particles", file=sys.stderr, flush=True)
-    b = BatchSim(sims, branches=min(4, len(sims)))
-    b.step(args.batch_warmup)
-    torch.cuda.synchronize()
-    ne = 10
-    t0 = time.perf_counter()
-    b.step_eager(ne)
-    torch.cuda.synchronize()
-    eager = (time.perf_counter() - t0) / ne
-    # 5 timed launches per candidate (32 candidates, state restored between
-    # them): 2 left the choice to launch-to-launch noise
-    tune = b.autotune(trials=5)
+    scs = [scenes.batch_box(side, seed=1 + k, dt=dt, g=g) for k, (side, _, _) in enumerate(sizes)]
+    fr = [(ms, mk) for _, ms, mk in sizes]
+    total = sum(sc.n for sc in scs)
+    cells = [sc.cells[0] * sc.cells[1] * sc.cells[2] for sc in scs]
+    bytes_step = sum(sc.n * bytes_per_particle_step(sc.n, c, sc.boundary.shape[0], 3, 13)
+                     for sc, c in zip(scs, cells))
+    print(f"# batch: {len(scs)} scenes, {total} particles", file=sys.stderr, flush=True)
+
+    # one set, one launch per stage
+    sb = SceneBatch(scs, fr, stab=3, solver=10)
+    sb.step(args.batch_warmup)
+    sb.capture(steps_per_launch=1)
+    tune1 = sb.autotune(trials=5)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    b.step(args.batch_steps)
+    sb.launch(args.batch_steps)
     torch.cuda.synchronize()
-    graph = (time.perf_counter() - t0) / args.batch_steps
-    for s in sims:
-        p, v, _ = s.host()
-        assert np.isfinite(p).all() and np.isfinite(v).all(), "non-finite batch state"
-    b.close()
-    print(f"# batch timed {args.batch_steps} launches: {1000 * graph:.3f} ms/step "
-          f"(eager {1000 * eager:.3f})", file=sys.stderr, flush=True)
-    ach = bytes_step / graph / 1e9
+    fused = (time.perf_counter() - t0) / args.batch_steps
+    assert torch.isfinite(sb.pos).all() and torch.isfinite(sb.vel).all(), "non-finite batch"
+    sb.close_graph()
+    del sb
+    torch.cuda.empty_cache()
+
+    # each scene's own step chain in one graph, and eagerly, over the same
+    # steps from fresh copies of the scenes
+    def per_scene(eager_mode):
+        sims = [PlainSim(sc, stab=3, solver=10, mu_s=ms, mu_k=mk)
+                for sc, (ms, mk) in zip(scs, fr)]
+        b = BatchSim(sims, branches=min(4, len(sims)))
+        b.step(args.batch_warmup)
+        tune = None if eager_mode else b.autotune(trials=5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if eager_mode:
+            b.step_eager(args.batch_steps)
+        else:
+            b.step(args.batch_steps)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / args.batch_steps
+        b.close()
+        del b, sims
+        torch.cuda.empty_cache()
+        return el, tune
+
+    graph, tune2 = per_scene(False)
+    eager, _ = per_scene(True)
+    print(f"# batch timed {args.batch_steps} launches: one set {1000 * fused:.3f} ms/step, "
+          f"per-scene graph {1000 * graph:.3f}, eager {1000 * eager:.3f}", file=sys.stderr,
+          flush=True)
+    ach = bytes_step / fused / 1e9
     return {
-        "workload": f"{len(sims)} plain box scenes, counts log-uniform in [4k, 256k] (seed 4), "
-                    f"friction mu_s U(0.3, 0.7), mu_k = 0.8 mu_s, 3 + 10 sweeps, dt = 0.002, "
+        "workload": f"{len(scs)} plain box scenes, counts log-uniform in [4k, 256k] (seed 4), "
+                    f"friction mu_s U(0.3, 0.7), mu_k = 0.8 mu_s, 3 + 10 sweeps, dt = {dt:.6g}, "
                     f"steps {args.batch_warmup}..{args.batch_warmup + args.batch_steps}",
-        "scenes": len(sims), "particles": total,
-        "particle_steps_per_s": round(total / graph, 1),
-        "ms_per_step": round(1000 * graph, 4),
+        "scenes": len(scs), "particles": total,
+        "particle_steps_per_s": round(total / fused, 1),
+        "ms_per_step": round(1000 * fused, 4),
+        "launch": "one set, one launch per stage, captured (gg_scenes_graph)",
+        "autotuned": tune1,
+        "per_scene_graph": {"particle_steps_per_s": round(total / graph, 1),
+                            "ms_per_step": round(1000 * graph, 4), "autotuned": tune2},
         "eager_particle_steps_per_s": round(total / eager, 1),
         "eager_ms_per_step": round(1000 * eager, 4),
-        "autotuned": tune,
         "step_hbm_model": {"bytes_per_step": int(bytes_step), "achieved_GBs": round(ach, 1),
                            "frac_of_hbm_roofline": round(ach / HBM_PEAK_GBS, 4)},
     }
@@ -402,6 +430,8 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-warmup", type=int, default=20)
+    ap.add_argument("--batch-dt", type=float, default=1.0 / 240.0,
+                    help="time step of the batched scenes (the plain box's 1/240)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 512 MiB device-to-device copy (calibrates HBM counters)")
     args = ap.parse_args()
