@@ -82,7 +82,8 @@ class gg_workspace(C.Structure):
 class gg_mlp_config(C.Structure):
     _fields_ = [(k, C.c_float) for k in ("radius", "vscale", "guard_boundary", "guard_floor",
                                          "guard_side")] + [("infer_waves", C.c_int),
-                                                           ("infer_tiles", C.c_int)]
+                                                           ("infer_tiles", C.c_int),
+                                                           ("select_chunk", C.c_int)]
 
 
 class gg_mlp_weights(C.Structure):

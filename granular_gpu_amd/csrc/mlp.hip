@@ -173,6 +173,17 @@ __device__ __forceinline__ void kv_sort16(unsigned long long (&v)[16]) {
   for (int k = 0; k < 63; ++k) kv_ce(v[kOddEvenMerge16[2 * k]], v[kOddEvenMerge16[2 * k + 1]]);
 }
 
+// Batcher's network for 8 keys (19 compare-exchanges), checked the same way
+// on all 2^8 zero-one inputs.
+constexpr unsigned char kOddEvenMerge8[38] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 2, 1, 3, 4,
+                                              6, 5, 7, 1, 2, 5, 6, 0, 4, 1, 5, 2, 6,
+                                              3, 7, 2, 4, 3, 5, 1, 2, 3, 4, 5, 6};
+
+__device__ __forceinline__ void kv_sort8(unsigned long long (&v)[8]) {
+#pragma unroll
+  for (int k = 0; k < 19; ++k) kv_ce(v[kOddEvenMerge8[2 * k]], v[kOddEvenMerge8[2 * k + 1]]);
+}
+
 // top <- the 16 smallest keys of top and c (both ascending), ascending:
 // min(top[i], c[15 - i]) holds them as a bitonic sequence, which a
 // bitonic merger sorts (16 minima + 32 compare-exchanges).
@@ -180,6 +191,21 @@ __device__ __forceinline__ void kv_merge16(unsigned long long (&top)[16],
                                            const unsigned long long (&c)[16]) {
 #pragma unroll
   for (int i = 0; i < 16; ++i) top[i] = c[15 - i] < top[i] ? c[15 - i] : top[i];
+#pragma unroll
+  for (int j = 8; j > 0; j >>= 1)
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int l = i ^ j;
+      if (l > i) kv_ce(top[i], top[l]);
+    }
+}
+
+// The same with 8 new keys (c padded with the largest key: the first 8
+// minima keep top[0..7]).
+__device__ __forceinline__ void kv_merge8(unsigned long long (&top)[16],
+                                          const unsigned long long (&c)[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) top[8 + i] = c[7 - i] < top[8 + i] ? c[7 - i] : top[8 + i];
 #pragma unroll
   for (int j = 8; j > 0; j >>= 1)
 #pragma unroll
@@ -200,7 +226,7 @@ __device__ __forceinline__ void kv_merge16(unsigned long long (&top)[16],
 // are contiguous).
 // FLAGS = 1 (inference): also the fallback flags and the compacted list of
 // the flagged particles (k_mlp_flags fused in).
-template <int GROUPED, int FLAGS>
+template <int GROUPED, int FLAGS, int CHUNK>
 __global__ void __launch_bounds__(256)
     k_mlp_features(int n, const float *__restrict__ pos, const float *__restrict__ vel,
                    const int *__restrict__ cs, DevGrid g, MlpConst c,
@@ -248,10 +274,10 @@ __global__ void __launch_bounds__(256)
   row_range(r, na, nb);
   int j = 0, j1 = 0;               // rest of the current row
   for (bool first = true;; first = false) {
-    // the next 16 candidates (index -1 past the last row)
-    int jj[kKnn];
+    // the next CHUNK candidates (index -1 past the last row)
+    int jj[CHUNK];
 #pragma unroll
-    for (int s = 0; s < kKnn; ++s) {
+    for (int s = 0; s < CHUNK; ++s) {
       while (j >= j1 && r < 9) {
         j = na;
         j1 = nb;
@@ -259,10 +285,10 @@ __global__ void __launch_bounds__(256)
       }
       jj[s] = j < j1 ? j++ : -1;
     }
-    unsigned long long c[kKnn];
+    unsigned long long c[CHUNK];
     bool need = false;
 #pragma unroll
-    for (int s = 0; s < kKnn; ++s) {
+    for (int s = 0; s < CHUNK; ++s) {
       const int js = jj[s];
       const Packed3 pj = ld3(pos, js >= 0 ? js : i);
       const float dx = pj.x - xx, dy = pj.y - xy, dz = pj.z - xz;
@@ -274,12 +300,17 @@ __global__ void __launch_bounds__(256)
     }
     if (first) {
       // nothing kept yet: the sorted chunk is the merge's result
-      kv_sort16(c);
+      if constexpr (CHUNK == 16) kv_sort16(c); else kv_sort8(c);
 #pragma unroll
-      for (int s = 0; s < kKnn; ++s) key[s] = c[s];
+      for (int s = 0; s < CHUNK; ++s) key[s] = c[s];
     } else if (__ballot(need)) {
-      kv_sort16(c);
-      kv_merge16(key, c);
+      if constexpr (CHUNK == 16) {
+        kv_sort16(c);
+        kv_merge16(key, c);
+      } else {
+        kv_sort8(c);
+        kv_merge8(key, c);
+      }
     }
     if (!__ballot(j < j1 || r < 9)) break;
   }
@@ -874,6 +905,7 @@ extern "C" void gg_mlp_config_default(gg_mlp_config *c) {
   c->guard_side = 0.05f;
   c->infer_waves = 0;
   c->infer_tiles = 0;
+  c->select_chunk = 0;
 }
 
 extern "C" int gg_mlp_features(const gg_particles *coarse, const int *cs,
@@ -890,7 +922,7 @@ extern "C" int gg_mlp_features(const gg_particles *coarse, const int *cs,
       reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), mlp_const(c), flags,
       nullptr, nullptr);
   GG_LAUNCH_CHECK();
-  k_mlp_features<0, 0><<<nblocks(coarse->n, 256), 256, 0, stream>>>(
+  k_mlp_features<0, 0, 16><<<nblocks(coarse->n, 256), 256, 0, stream>>>(
       coarse->n, reinterpret_cast<const float *>(coarse->pos),
       reinterpret_cast<const float *>(coarse->vel), cs, dev_grid(grid), mlp_const(c), feat, knn,
       nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -1100,9 +1132,14 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   prof_begin(GG_STAGE_MLP_FEATURES, stream);
   k_zero_int<<<1, 1, 0, stream>>>(fb_count);
   GG_LAUNCH_CHECK();
-  k_mlp_features<1, 1><<<nblocks(n, 256), 256, 0, stream>>>(
-      n, pos, vel, cs, g, mlp_const(c), feat, nullptr,
-      reinterpret_cast<const float *>(boundary->pos), bcs, flags, fb_list, fb_count);
+  if (c->select_chunk != 16)
+    k_mlp_features<1, 1, 8><<<nblocks(n, 256), 256, 0, stream>>>(
+        n, pos, vel, cs, g, mlp_const(c), feat, nullptr,
+        reinterpret_cast<const float *>(boundary->pos), bcs, flags, fb_list, fb_count);
+  else
+    k_mlp_features<1, 1, 16><<<nblocks(n, 256), 256, 0, stream>>>(
+        n, pos, vel, cs, g, mlp_const(c), feat, nullptr,
+        reinterpret_cast<const float *>(boundary->pos), bcs, flags, fb_list, fb_count);
   GG_LAUNCH_CHECK();
   SideStream *side = nullptr;
   GG_TRY(side_stream(side));

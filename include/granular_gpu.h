@@ -393,6 +393,9 @@ typedef struct gg_mlp_config {
   int infer_waves;       /* inference: waves per block, 4, 8 or 16 (0 = 8)  */
   int infer_tiles;       /* inference: 16-sample tiles per wave, 1, 2 or 4
                             (0 = 4); 16 waves only with 1 tile             */
+  int select_chunk;      /* inference features: candidates sorted and merged
+                            per step of the kNN selection, 8 or 16 (0 = 8);
+                            the selected set does not depend on it         */
 } gg_mlp_config;
 typedef struct gg_mlp_weights {
   uint16_t *w_bf16;  /* GG_MLP_WEIGHTS bfloat16 (inference and training)  */

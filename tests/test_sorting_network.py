@@ -44,3 +44,44 @@ def test_sorts_keys_with_ties():
             if v[b] < v[a]:
                 v[a], v[b] = v[b], v[a]
         assert v == sorted(keys)
+
+
+def network8():
+    m = re.search(r"kOddEvenMerge8\[38\]\s*=\s*\{([^}]*)\}", SRC.read_text())
+    vals = [int(v) for v in m.group(1).replace("\n", " ").split(",") if v.strip()]
+    assert len(vals) == 38
+    return list(zip(vals[0::2], vals[1::2]))
+
+
+def test_eight_key_network_sorts_every_zero_one_input():
+    net = network8()
+    assert len(net) == 19 and all(0 <= a < b < 8 for a, b in net)
+    v = ((np.arange(1 << 8)[:, None] >> np.arange(8)[None, :]) & 1).astype(np.int8)
+    for a, b in net:
+        lo = np.minimum(v[:, a], v[:, b])
+        hi = np.maximum(v[:, a], v[:, b])
+        v[:, a], v[:, b] = lo, hi
+    assert (np.diff(v, axis=1) >= 0).all()
+
+
+def test_merge_of_eight_keeps_the_sixteen_smallest():
+    # the merge step of the 8-key selection: minima against the reversed
+    # padded chunk, then the bitonic merger, on random keys with ties
+    rng = np.random.default_rng(2)
+    none = np.uint64(2 ** 64 - 1)
+    for _ in range(300):
+        top = np.sort(rng.integers(0, 1 << 20, 16, dtype=np.uint64))
+        top[rng.random(16) < 0.2] = none
+        top = np.sort(top)
+        c = np.sort(rng.integers(0, 1 << 20, 8, dtype=np.uint64))
+        t = list(top)
+        for i in range(8):
+            t[8 + i] = min(t[8 + i], c[7 - i])
+        j = 8
+        while j > 0:
+            for i in range(16):
+                l = i ^ j
+                if l > i and t[l] < t[i]:
+                    t[i], t[l] = t[l], t[i]
+            j >>= 1
+        assert t == sorted(np.concatenate([top, c]))[:16]
