@@ -172,7 +172,11 @@ def upsample_phase(args, lib, torch, C, np):
             pmc_up = {}
 
     def traffic_of(kernel):
-        return pmc_up.get(kernel, {}).get("hbm_bytes_per_launch")
+        # by name prefix: template arguments follow the launch configuration
+        for k, v in pmc_up.items():
+            if k.startswith(kernel):
+                return v.get("hbm_bytes_per_launch")
+        return None
     print(f"# upsample timed {args.upsample_steps} steps in {el:.3f} s", file=sys.stderr,
           flush=True)
     learned_out = None
@@ -226,7 +230,7 @@ def upsample_phase(args, lib, torch, C, np):
             "roofline": {"kernel": "k_mlp_infer (fused MFMA network)", "bound": "mfma",
                          "achieved": round(ach_l, 2), "peak": 2500.0, "unit": "TFLOP/s",
                          "frac": round(ach_l / 2500.0, 4),
-                         "traffic": traffic_of("void gg::k_mlp_infer<8, 4>"),
+                         "traffic": traffic_of("void gg::k_mlp_infer<16, 1>"),
                          "algorithmic_flops_per_launch": int(flops), "avg_launch_us": t_net},
         }
         print(f"# learned: {1e-9 * nf / (t_all * 1e-6):.2f} G fine/s, error {e_l:.4f} r_fine "

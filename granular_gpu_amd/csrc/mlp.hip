@@ -224,26 +224,15 @@ __device__ __forceinline__ void kv_merge8(unsigned long long (&top)[16],
 // GROUPED = 1: feat[g][i][4] for the 24 groups g of 4 consecutive features
 // (the inference layout: a wave's stores and the network's B-operand loads
 // are contiguous).
-// FLAGS = 1 (inference): also the fallback flags and the compacted list of
-// the flagged particles (k_mlp_flags fused in).
-template <int GROUPED, int FLAGS, int CHUNK>
+// CHUNK: candidates per selection step (16, or 8 with fewer registers:
+// 96 instead of 114, 5 waves per SIMD instead of 4).
+template <int GROUPED, int CHUNK>
 __global__ void __launch_bounds__(256)
     k_mlp_features(int n, const float *__restrict__ pos, const float *__restrict__ vel,
                    const int *__restrict__ cs, DevGrid g, MlpConst c,
-                   uint16_t *__restrict__ feat, int *__restrict__ knn,
-                   const float *__restrict__ bpos, const int *__restrict__ bcs,
-                   int *__restrict__ flags, int *__restrict__ fb_list,
-                   int *__restrict__ fb_count) {
+                   uint16_t *__restrict__ feat, int *__restrict__ knn) {
   // consecutive blocks stay on one XCD (spatial neighbours share its L2)
   const int i = xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x;
-  if (FLAGS) {
-    int flag = 0;
-    if (i < n) {
-      flag = guard_flag(ld3(pos, i), bpos, bcs, g, c);
-      flags[i] = flag;
-    }
-    list_flagged(i, flag, fb_list, fb_count);
-  }
   if (i >= n) return;
   const Packed3 xi = ld3(pos, i);
   const float xx = xi.x, xy = xi.y, xz = xi.z;
@@ -922,10 +911,9 @@ extern "C" int gg_mlp_features(const gg_particles *coarse, const int *cs,
       reinterpret_cast<const float *>(boundary->pos), bcs, dev_grid(grid), mlp_const(c), flags,
       nullptr, nullptr);
   GG_LAUNCH_CHECK();
-  k_mlp_features<0, 0, 16><<<nblocks(coarse->n, 256), 256, 0, stream>>>(
+  k_mlp_features<0, 16><<<nblocks(coarse->n, 256), 256, 0, stream>>>(
       coarse->n, reinterpret_cast<const float *>(coarse->pos),
-      reinterpret_cast<const float *>(coarse->vel), cs, dev_grid(grid), mlp_const(c), feat, knn,
-      nullptr, nullptr, nullptr, nullptr, nullptr);
+      reinterpret_cast<const float *>(coarse->vel), cs, dev_grid(grid), mlp_const(c), feat, knn);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
@@ -1104,8 +1092,10 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   const int n = coarse->n;
   if (n < 0 || grid.cx <= 0 || grid.cy <= 0 || grid.cz <= 0) return GG_ERR_INVALID;
   if (!(c->radius > 0.0f) || !(c->vscale > 0.0f)) return GG_ERR_INVALID;
-  const int waves = c->infer_waves == 0 ? 8 : c->infer_waves;
-  const int tiles = c->infer_tiles == 0 ? 4 : c->infer_tiles;
+  // default 16 waves of one tile: at 250k and 1M coarse particles 4-6%
+  // faster than 8 waves of 4 tiles (more waves hide the feature loads)
+  const int waves = c->infer_waves == 0 ? 16 : c->infer_waves;
+  const int tiles = c->infer_tiles == 0 ? 1 : c->infer_tiles;
   // (16 waves, 2 or 4 tiles) would spill registers: not offered
   if (!(((waves == 4 || waves == 8) && (tiles == 1 || tiles == 2 || tiles == 4)) ||
         (waves == 16 && tiles == 1)))
@@ -1126,20 +1116,17 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const uint2 *feat4 = reinterpret_cast<const uint2 *>(feat);
-  // features with the flags and the compacted fallback list fused in, then
-  // the teacher on the flagged particles (the plain upsampler on the
-  // template) on the side stream beside the network (disjoint outputs)
+  // flags and the compacted fallback list first, so that the teacher on the
+  // flagged particles (the plain upsampler on the template) runs on the side
+  // stream beside the features and the network (they write disjoint
+  // outputs); fused into the features pass instead, the fallback would
+  // start only after it and overlap the network alone
   prof_begin(GG_STAGE_MLP_FEATURES, stream);
   k_zero_int<<<1, 1, 0, stream>>>(fb_count);
   GG_LAUNCH_CHECK();
-  if (c->select_chunk != 16)
-    k_mlp_features<1, 1, 8><<<nblocks(n, 256), 256, 0, stream>>>(
-        n, pos, vel, cs, g, mlp_const(c), feat, nullptr,
-        reinterpret_cast<const float *>(boundary->pos), bcs, flags, fb_list, fb_count);
-  else
-    k_mlp_features<1, 1, 16><<<nblocks(n, 256), 256, 0, stream>>>(
-        n, pos, vel, cs, g, mlp_const(c), feat, nullptr,
-        reinterpret_cast<const float *>(boundary->pos), bcs, flags, fb_list, fb_count);
+  k_mlp_flags<<<nblocks(n, 256), 256, 0, stream>>>(
+      n, pos, reinterpret_cast<const float *>(boundary->pos), bcs, g, mlp_const(c), flags,
+      fb_list, fb_count);
   GG_LAUNCH_CHECK();
   SideStream *side = nullptr;
   GG_TRY(side_stream(side));
@@ -1153,6 +1140,18 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   GG_LAUNCH_CHECK();
   prof_end(GG_STAGE_MLP_FALLBACK, side->st);
   if (hipEventRecord(side->join, side->st) != hipSuccess) return GG_ERR_HIP;
+  int frc = GG_OK;
+  if (c->select_chunk != 16)
+    k_mlp_features<1, 8><<<nblocks(n, 256), 256, 0, stream>>>(n, pos, vel, cs, g, mlp_const(c),
+                                                              feat, nullptr);
+  else
+    k_mlp_features<1, 16><<<nblocks(n, 256), 256, 0, stream>>>(n, pos, vel, cs, g,
+                                                               mlp_const(c), feat, nullptr);
+  if (hipGetLastError() != hipSuccess) frc = GG_ERR_HIP;
+  if (frc != GG_OK) {
+    (void)hipStreamWaitEvent(stream, side->join, 0);
+    return frc;
+  }
   prof_end(GG_STAGE_MLP_FEATURES, stream);
   prof_begin(GG_STAGE_MLP, stream);
   int rc;

@@ -390,9 +390,9 @@ typedef struct gg_mlp_config {
   float guard_boundary;  /* 0.05                                  */
   float guard_floor;     /* 0.08                                  */
   float guard_side;      /* 0.05                                  */
-  int infer_waves;       /* inference: waves per block, 4, 8 or 16 (0 = 8)  */
+  int infer_waves;       /* inference: waves per block, 4, 8 or 16 (0 = 16) */
   int infer_tiles;       /* inference: 16-sample tiles per wave, 1, 2 or 4
-                            (0 = 4); 16 waves only with 1 tile             */
+                            (0 = 1); 16 waves only with 1 tile             */
   int select_chunk;      /* inference features: candidates sorted and merged
                             per step of the kNN selection, 8 or 16 (0 = 8);
                             the selected set does not depend on it         */
