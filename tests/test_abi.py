@@ -115,3 +115,54 @@ def test_cpp_headers_compile_standalone(tmp_path):
         src.write_text(f'#include "granular/{h.name}"\nint main() {{ return 0; }}\n')
         subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-D__HIP_PLATFORM_AMD__",
                         "-I/opt/rocm/include", f"-I{ROOT / 'include'}", str(src)], check=True)
+
+
+def test_scenes_and_fine_targets_reject_bad_arguments_on_host():
+    # the multi-scene step, its graph and the fine-run targets check their
+    # arguments (the scene table against the workspace's cell count) before
+    # any device work
+    import ctypes as C
+    from granular_gpu_amd import _lib
+    lib = _lib.load()
+    g = _lib.gg_grid(0.0404, 9, 1, 1)  # T - 1 = 9 cells: two 2x2x1 scenes (4 + 1 each)
+    ws = _lib.gg_workspace(C.c_void_p(16).value, 1 << 30, 10, 10, g, 8, 48, 16)
+    desc = (_lib.gg_scene_desc * 2)(_lib.gg_scene_desc(2, 2, 1, 0, 0.5, 0.4),
+                                    _lib.gg_scene_desc(2, 2, 1, 5, 0.5, 0.4))
+    sc = _lib.gg_scenes(2, 16, C.addressof(desc), 16, 16, 1)
+    sset = _lib.gg_particles()
+    bset = _lib.gg_particles()
+    p = _lib.default_params()
+    bad = _lib.GG_ERR_INVALID if hasattr(_lib, "GG_ERR_INVALID") else -1
+    # a scene table whose cell ranges do not add up to the workspace's
+    desc[1].cell_base = 4
+    assert lib.gg_step_scenes(C.byref(sset), C.c_void_p(16), C.byref(bset), C.c_void_p(16),
+                              C.byref(sc), C.byref(p), C.byref(ws), None) == bad
+    desc[1].cell_base = 5
+    desc[1].cx = 3  # 3x2x1 + 1 overflows the 10 cells
+    assert lib.gg_scenes_boundary_build(C.byref(bset), C.c_void_p(16), C.byref(sc), C.byref(ws),
+                                        None) == bad
+    desc[1].cx = 2
+    desc[0].mu_s = -1.0  # negative friction
+    h = C.c_void_p()
+    assert lib.gg_scenes_graph_create(C.byref(sset), C.c_void_p(16), C.byref(bset),
+                                      C.c_void_p(16), C.byref(sc), C.byref(p), C.byref(ws), 1,
+                                      C.byref(h)) == bad
+    assert not h.value
+    assert lib.gg_scenes_graph_launch(None, None) == bad
+    lib.gg_scenes_graph_destroy(None)
+    # no host table
+    sc2 = _lib.gg_scenes(2, 16, None, 16, 16, 1)
+    assert lib.gg_step_scenes(C.byref(sset), C.c_void_p(16), C.byref(bset), C.c_void_p(16),
+                              C.byref(sc2), C.byref(p), C.byref(ws), None) == bad
+    # fine-run targets: missing outputs, bad grid, bad radius
+    cfg = _lib.gg_mlp_config()
+    lib.gg_mlp_config_default(C.byref(cfg))
+    fg = _lib.gg_grid(0.0404, 4, 4, 4)
+    assert lib.gg_mlp_fine_targets(C.byref(sset), C.byref(sset), C.c_void_p(16), fg,
+                                   C.byref(cfg), None, None, None, None) == bad
+    assert lib.gg_mlp_fine_targets(C.byref(sset), C.byref(sset), C.c_void_p(16),
+                                   _lib.gg_grid(0.0, 4, 4, 4), C.byref(cfg), C.c_void_p(16),
+                                   None, None, None) == bad
+    cfg.radius = 0.0
+    assert lib.gg_mlp_fine_targets(C.byref(sset), C.byref(sset), C.c_void_p(16), fg,
+                                   C.byref(cfg), C.c_void_p(16), None, None, None) == bad
