@@ -601,7 +601,18 @@ __device__ __forceinline__ const bf16x8 &frag(const uint16_t *__restrict__ W, in
   return *reinterpret_cast<const bf16x8 *>(W + ((t * ksteps + ks) * 64 + lane) * 8);
 }
 
-// Copies one layer's row-major weights (rows x 32*ksteps) into fragment order.
+// The output layer's rows in the inference kernel: row 16 t + 4 q + e holds
+// output 3 (4 t + q) + e (slot 4 t + q, axis e) for e < 3 and zeros for
+// e = 3, so that lane (c, q) of output tile t ends with the three velocity
+// offsets of one fine particle.  -1: a zero row.
+__device__ __forceinline__ int out_row_source(int row) {
+  const int e = row & 3, slot = 4 * (row >> 4) + ((row >> 2) & 3);
+  return e < 3 ? 3 * slot + e : -1;
+}
+
+// Copies one layer's row-major weights (rows x 32*ksteps) into fragment order;
+// OUT_ROWS: the rows of the output layer reordered by out_row_source.
+template <bool OUT_ROWS = false>
 __device__ __forceinline__ void stage_layer(uint16_t *__restrict__ dst,
                                             const uint16_t *__restrict__ src, int rows,
                                             int ksteps) {
@@ -610,9 +621,13 @@ __device__ __forceinline__ void stage_layer(uint16_t *__restrict__ dst,
     const int lane = f & 63, fk = f >> 6;
     const int t = fk / ksteps, ks = fk % ksteps;
     const int o = 16 * t + (lane & 15), k = 32 * ks + 4 * (lane >> 4);
-    const uint16_t *row = src + (size_t)o * (32 * ksteps);
-    const uint2 lo = *reinterpret_cast<const uint2 *>(row + k);
-    const uint2 hi = *reinterpret_cast<const uint2 *>(row + k + 16);
+    const int so = OUT_ROWS ? out_row_source(o) : o;
+    uint2 lo = make_uint2(0u, 0u), hi = lo;
+    if (so >= 0) {
+      const uint16_t *row = src + (size_t)so * (32 * ksteps);
+      lo = *reinterpret_cast<const uint2 *>(row + k);
+      hi = *reinterpret_cast<const uint2 *>(row + k + 16);
+    }
     reinterpret_cast<uint4 *>(dst)[f] = make_uint4(lo.x, lo.y, hi.x, hi.y);
   }
 }
@@ -621,35 +636,60 @@ __device__ __forceinline__ void stage_layer(uint16_t *__restrict__ dst,
 // sample tile from KSTEPS 32-wide k-steps, accumulated onto the bias; ReLU
 // and hardware bfloat16 rounding give the next layer's operand halves
 // (out[s][t] = features 16t + 4q .. + 3 of lane (c, q)'s sample).
+// ReLU (no NaN survives it), then hardware round-to-nearest-even.
+__device__ __forceinline__ bf16x4 relu_bf16x4(const f32x4 &z) {
+  const u32x2v h = {pack_bf16x2(z[0] > 0.0f ? z[0] : 0.0f, z[1] > 0.0f ? z[1] : 0.0f),
+                    pack_bf16x2(z[2] > 0.0f ? z[2] : 0.0f, z[3] > 0.0f ? z[3] : 0.0f)};
+  return __builtin_bit_cast(bf16x4, h);
+}
+
+// Software-pipelined over the 8 output tiles: each tile's weight fragments
+// and bias are read from LDS one tile ahead, and its ReLU runs after the
+// next tile's MFMAs are issued, so that neither the LDS latency nor the
+// MFMA result latency stalls the wave.
 template <int TILES, int KSTEPS>
 __device__ __forceinline__ void hidden_layer(const uint16_t *__restrict__ W,
                                              const float *__restrict__ bias,
                                              const bf16x8 (&in)[TILES][4],
                                              bf16x4 (&out)[TILES][8], int lane) {
   const int q = lane >> 4;
+  bf16x8 w[KSTEPS];
+  f32x4 b = *reinterpret_cast<const f32x4 *>(bias + 4 * q);
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ++ks) w[ks] = frag(W, KSTEPS, 0, ks, lane);
+  f32x4 prev[TILES];
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    const f32x4 b = *reinterpret_cast<const f32x4 *>(bias + 16 * t + 4 * q);
+    bf16x8 wn[KSTEPS];
+    f32x4 bn = b;
+    if (t + 1 < 8) {
+      bn = *reinterpret_cast<const f32x4 *>(bias + 16 * (t + 1) + 4 * q);
+#pragma unroll
+      for (int ks = 0; ks < KSTEPS; ++ks) wn[ks] = frag(W, KSTEPS, t + 1, ks, lane);
+    }
     f32x4 a[TILES];
 #pragma unroll
     for (int s = 0; s < TILES; ++s) a[s] = b;
 #pragma unroll
-    for (int ks = 0; ks < KSTEPS; ++ks) {
-      const bf16x8 w = frag(W, KSTEPS, t, ks, lane);
+    for (int ks = 0; ks < KSTEPS; ++ks)
 #pragma unroll
-      for (int s = 0; s < TILES; ++s) a[s] = mfma32(w, in[s][ks], a[s]);
-    }
-    // ReLU (no NaN survives it), then hardware round-to-nearest-even
+      for (int s = 0; s < TILES; ++s) a[s] = mfma32(w[ks], in[s][ks], a[s]);
+    if (t > 0) {
 #pragma unroll
-    for (int s = 0; s < TILES; ++s) {
-      const f32x4 z = a[s];
-      const u32x2v h = {pack_bf16x2(z[0] > 0.0f ? z[0] : 0.0f, z[1] > 0.0f ? z[1] : 0.0f),
-                        pack_bf16x2(z[2] > 0.0f ? z[2] : 0.0f, z[3] > 0.0f ? z[3] : 0.0f)};
-      out[s][t] = __builtin_bit_cast(bf16x4, h);
+      for (int s = 0; s < TILES; ++s) out[s][t - 1] = relu_bf16x4(prev[s]);
     }
-    // keep each tile's weight loads next to its MFMAs (bounds live registers)
+#pragma unroll
+    for (int s = 0; s < TILES; ++s) prev[s] = a[s];
+    if (t + 1 < 8) {
+      b = bn;
+#pragma unroll
+      for (int ks = 0; ks < KSTEPS; ++ks) w[ks] = wn[ks];
+    }
+    // one tile in flight at a time (bounds live registers)
     __builtin_amdgcn_sched_barrier(0);
   }
+#pragma unroll
+  for (int s = 0; s < TILES; ++s) out[s][7] = relu_bf16x4(prev[s]);
 }
 
 // The next layer's k-step operands from a hidden layer's output tiles.
@@ -661,7 +701,8 @@ __device__ __forceinline__ void operands(const bf16x4 (&h)[TILES][8], bf16x8 (&i
     for (int ks = 0; ks < 4; ++ks) in[s][ks] = cat8(h[s][2 * ks], h[s][2 * ks + 1]);
 }
 
-// Output layer: 2 tiles (24 outputs + 8 zero rows) onto the bias, no activation.
+// Output layer: 2 tiles (24 outputs + 8 zero rows, in out_row_source's
+// order) onto the bias, no activation.
 template <int TILES>
 __device__ __forceinline__ void output_layer(const uint16_t *__restrict__ W,
                                              const float *__restrict__ bias,
@@ -702,18 +743,39 @@ __device__ __forceinline__ void load_features(int n, const uint2 *__restrict__ f
   }
 }
 
+// Flags, positions and velocities of sample group grp (index clamped to the
+// last sample: the loads need no branch).
+template <int TILES>
+__device__ __forceinline__ void load_samples(int n, const int *__restrict__ flags,
+                                             const float *__restrict__ pos,
+                                             const float *__restrict__ vel, int grp, int c,
+                                             int (&fl)[TILES], Packed3 (&ps)[TILES],
+                                             Packed3 (&vs)[TILES]) {
+#pragma unroll
+  for (int s = 0; s < TILES; ++s) {
+    const int smp = grp * 16 * TILES + 16 * s + c;
+    const int ld = smp < n ? smp : n - 1;
+    fl[s] = flags[ld];
+    ps[s] = ld3(pos, ld);
+    vs[s] = ld3(vel, ld);
+  }
+}
+
 // feat4: grouped features, feat4[g * n + i] = features 4g..4g+3 of sample i.
-// Each wave loads the next group's features before computing the current
-// one, so their latency hides behind the network.  vec4: fine_pos and
-// fine_vel are 16-byte aligned, and each lane stores its 4 consecutive
-// outputs of a sample (16t + 4q .. + 3) as one 16-byte store.
+// Each wave loads the next group's features, flags, positions and
+// velocities before computing the current group, so that their latency
+// hides behind the network (loads complete in issue order: the current
+// group's inputs, loaded one iteration earlier, are waited for without
+// waiting for the next group's).  With the output
+// rows reordered, lane (c, q) of output tile t holds the velocity offsets of
+// fine particle 8 i + 4 t + q (i its sample), stored as one position and one
+// velocity triple.
 template <int WAVES, int TILES>
 __global__ void __launch_bounds__(64 * WAVES)
     k_mlp_infer(int n, const uint2 *__restrict__ feat4, const int *__restrict__ flags,
                 const float *__restrict__ pos, const float *__restrict__ vel,
                 const uint16_t *__restrict__ wts, const float *__restrict__ bias_g, float radius,
-                float dt, float *__restrict__ fine_pos, float *__restrict__ fine_vel,
-                int vec4) {
+                float dt, float *__restrict__ fine_pos, float *__restrict__ fine_vel) {
   __shared__ __attribute__((aligned(16))) uint16_t W[49152];
   __shared__ __attribute__((aligned(16))) float bias[416];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -723,17 +785,33 @@ __global__ void __launch_bounds__(64 * WAVES)
   const int stride = gridDim.x * WAVES;
   int grp = blockIdx.x * WAVES + wave;
   uint4 cur[TILES][3];
+  int fl[TILES];
+  Packed3 ps[TILES], vs[TILES];
   load_features<TILES>(n, feat4, grp, groups, c, q, cur);
+  load_samples<TILES>(n, flags, pos, vel, grp, c, fl, ps, vs);
   stage_layer(W + kWOff1, wts + kWOff1, kHid, kIn / 32);
   stage_layer(W + kWOff2, wts + kWOff2, kHid, kHid / 32);
   stage_layer(W + kWOff3, wts + kWOff3, kHid, kHid / 32);
-  stage_layer(W + kWOff4, wts + kWOff4, kOutPad, kHid / 32);
-  for (int i = threadIdx.x; i < 416; i += blockDim.x) bias[i] = bias_g[i];
+  stage_layer<true>(W + kWOff4, wts + kWOff4, kOutPad, kHid / 32);
+  for (int i = threadIdx.x; i < 416; i += blockDim.x) {
+    const int so = i < kBOff4 ? i : out_row_source(i - kBOff4);
+    bias[i] = so < 0 ? 0.0f : (i < kBOff4 ? bias_g[i] : bias_g[kBOff4 + so]);
+  }
+  // the template offsets radius * T of this lane's two slots q and 4 + q
+  float rt[2][3];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) rt[t][ax] = radius * templ(4 * t + q, ax);
   __syncthreads();
   for (; grp < groups; grp += stride) {
     const int base = grp * per_iter;
     uint4 nxt[TILES][3];
+    int fl_n[TILES];
+    Packed3 ps_n[TILES], vs_n[TILES];
     load_features<TILES>(n, feat4, grp + stride, groups, c, q, nxt);
+    load_samples<TILES>(n, flags, pos, vel, grp + stride, c, fl_n, ps_n, vs_n);
+    __builtin_amdgcn_sched_barrier(0);
     bf16x8 in[TILES][4];
     bf16x4 h[TILES][8];
     f32x4 y[TILES][2];
@@ -750,44 +828,36 @@ __global__ void __launch_bounds__(64 * WAVES)
     hidden_layer<TILES, 4>(W + kWOff3, bias + kBOff3, in, h, lane);
     operands<TILES>(h, in);
     output_layer<TILES>(W + kWOff4, bias + kBOff4, in, y, lane);
-    // outputs f = 16 t + 4 q + e of sample base + 16 s + c
+    // fine particle 8 smp + 4 t + q of sample smp = base + 16 s + c (computed
+    // on every lane, stored for the samples that exist and are not flagged)
 #pragma unroll
     for (int s = 0; s < TILES; ++s) {
       const int smp = base + 16 * s + c;
-      if (smp >= n || flags[smp] != 0) continue;
-      const float px = pos[3 * smp], py = pos[3 * smp + 1], pz = pos[3 * smp + 2];
-      const float vx = vel[3 * smp], vy = vel[3 * smp + 1], vz = vel[3 * smp + 2];
+      const bool store = smp < n && fl[s] == 0;
+      const Packed3 p = ps[s], v = vs[s];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        if (16 * t + 4 * q >= kOut) continue;
-        float fp[4], fv[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int f = 16 * t + 4 * q + e;
-          const int k = f / 3, ax = f % 3;
-          const float p0 = ax == 0 ? px : (ax == 1 ? py : pz);
-          const float v0 = ax == 0 ? vx : (ax == 1 ? vy : vz);
-          fv[e] = v0 + y[s][t][e];
-          fp[e] = (p0 + radius * templ(k, ax)) + dt * fv[e];
-        }
-        const size_t o = (size_t)smp * kOut + 16 * t + 4 * q;
-        if (vec4) {
-          *reinterpret_cast<float4 *>(fine_pos + o) = make_float4(fp[0], fp[1], fp[2], fp[3]);
-          if (fine_vel)
-            *reinterpret_cast<float4 *>(fine_vel + o) = make_float4(fv[0], fv[1], fv[2], fv[3]);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            fine_pos[o + e] = fp[e];
-            if (fine_vel) fine_vel[o + e] = fv[e];
-          }
+        const float fvx = v.x + y[s][t][0];
+        const float fvy = v.y + y[s][t][1];
+        const float fvz = v.z + y[s][t][2];
+        const float fpx = (p.x + rt[t][0]) + dt * fvx;
+        const float fpy = (p.y + rt[t][1]) + dt * fvy;
+        const float fpz = (p.z + rt[t][2]) + dt * fvz;
+        if (store) {
+          const long long fi = 8ll * smp + 4 * t + q;
+          st3(fine_pos, fi, fpx, fpy, fpz);
+          if (fine_vel) st3(fine_vel, fi, fvx, fvy, fvz);
         }
       }
     }
 #pragma unroll
-    for (int s = 0; s < TILES; ++s)
+    for (int s = 0; s < TILES; ++s) {
 #pragma unroll
       for (int ks = 0; ks < 3; ++ks) cur[s][ks] = nxt[s][ks];
+      fl[s] = fl_n[s];
+      ps[s] = ps_n[s];
+      vs[s] = vs_n[s];
+    }
   }
 }
 
@@ -1048,9 +1118,8 @@ static int launch_infer(int n, const uint2 *feat4, const int *flags, const float
   const int groups = (n + 16 * TILES - 1) / (16 * TILES);
   int blocks = (groups + WAVES - 1) / WAVES;
   if (blocks > cus) blocks = cus;
-  const int vec4 = ((uintptr_t)fine_pos % 16 == 0 && (uintptr_t)fine_vel % 16 == 0) ? 1 : 0;
-  k_mlp_infer<WAVES, TILES><<<blocks, 64 * WAVES, 0, st>>>(n, feat4, flags, pos, vel, w, bias,
-                                                          radius, dt, fine_pos, fine_vel, vec4);
+  k_mlp_infer<WAVES, TILES><<<blocks, 64 * WAVES, 0, st>>>(
+      n, feat4, flags, pos, vel, w, bias, radius, dt, fine_pos, fine_vel);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
