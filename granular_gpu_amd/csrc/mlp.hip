@@ -1130,7 +1130,7 @@ static int launch_infer(int n, const uint2 *feat4, const int *flags, const float
 // kept for the life of the process.
 struct SideStream {
   hipStream_t st = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, flags = nullptr;
 };
 static int side_stream(SideStream *&out) {
   static std::mutex mu;
@@ -1142,7 +1142,8 @@ static int side_stream(SideStream *&out) {
   if (!s.st) {
     if (hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking) != hipSuccess) return GG_ERR_HIP;
     if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.flags, hipEventDisableTiming) != hipSuccess)
       return GG_ERR_HIP;
   }
   out = &s;
@@ -1185,30 +1186,36 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const uint2 *feat4 = reinterpret_cast<const uint2 *>(feat);
-  // flags and the compacted fallback list first, so that the teacher on the
-  // flagged particles (the plain upsampler on the template) runs on the side
-  // stream beside the features and the network (they write disjoint
-  // outputs); fused into the features pass instead, the fallback would
-  // start only after it and overlap the network alone
-  prof_begin(GG_STAGE_MLP_FEATURES, stream);
-  k_zero_int<<<1, 1, 0, stream>>>(fb_count);
-  GG_LAUNCH_CHECK();
-  k_mlp_flags<<<nblocks(n, 256), 256, 0, stream>>>(
-      n, pos, reinterpret_cast<const float *>(boundary->pos), bcs, g, mlp_const(c), flags,
-      fb_list, fb_count);
-  GG_LAUNCH_CHECK();
+  // The side stream computes the flags and the compacted list of flagged
+  // particles, then the teacher on them (the plain upsampler on the
+  // template), beside the features pass and the network on the caller's
+  // stream (disjoint outputs); the network waits for the flags only.  Fused
+  // into the features pass instead, the fallback would start only after it.
   SideStream *side = nullptr;
   GG_TRY(side_stream(side));
   if (hipEventRecord(side->fork, stream) != hipSuccess ||
       hipStreamWaitEvent(side->st, side->fork, 0) != hipSuccess)
     return GG_ERR_HIP;
   prof_begin(GG_STAGE_MLP_FALLBACK, side->st);
-  k_mlp_fallback<<<cus * 8, 256, 0, side->st>>>(
-      fb_list, fb_count, pos, vel, cs, reinterpret_cast<const float *>(boundary->pos), bcs, g,
-      up_const(u), c->radius, fine_pos, fine_vel);
-  GG_LAUNCH_CHECK();
+  k_zero_int<<<1, 1, 0, side->st>>>(fb_count);
+  k_mlp_flags<<<nblocks(n, 256), 256, 0, side->st>>>(
+      n, pos, reinterpret_cast<const float *>(boundary->pos), bcs, g, mlp_const(c), flags,
+      fb_list, fb_count);
+  const bool flags_ok =
+      hipGetLastError() == hipSuccess && hipEventRecord(side->flags, side->st) == hipSuccess;
+  if (flags_ok)
+    k_mlp_fallback<<<cus * 8, 256, 0, side->st>>>(
+        fb_list, fb_count, pos, vel, cs, reinterpret_cast<const float *>(boundary->pos), bcs,
+        g, up_const(u), c->radius, fine_pos, fine_vel);
   prof_end(GG_STAGE_MLP_FALLBACK, side->st);
+  const bool side_ok = flags_ok && hipGetLastError() == hipSuccess;
+  // the caller's stream always joins the side stream before returning
   if (hipEventRecord(side->join, side->st) != hipSuccess) return GG_ERR_HIP;
+  if (!side_ok) {
+    (void)hipStreamWaitEvent(stream, side->join, 0);
+    return GG_ERR_HIP;
+  }
+  prof_begin(GG_STAGE_MLP_FEATURES, stream);
   int frc = GG_OK;
   if (c->select_chunk != 16)
     k_mlp_features<1, 8><<<nblocks(n, 256), 256, 0, stream>>>(n, pos, vel, cs, g, mlp_const(c),
@@ -1217,6 +1224,7 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
     k_mlp_features<1, 16><<<nblocks(n, 256), 256, 0, stream>>>(n, pos, vel, cs, g,
                                                                mlp_const(c), feat, nullptr);
   if (hipGetLastError() != hipSuccess) frc = GG_ERR_HIP;
+  if (frc == GG_OK && hipStreamWaitEvent(stream, side->flags, 0) != hipSuccess) frc = GG_ERR_HIP;
   if (frc != GG_OK) {
     (void)hipStreamWaitEvent(stream, side->join, 0);
     return frc;
