@@ -23,6 +23,7 @@ GG_FLAG_NO_NLIST = 1
 GG_FLAG_NO_XCD_REMAP = 2
 GG_FLAG_FALLBACK_LANE = 4
 GG_FLAG_FALLBACK_WAVE = 8
+GG_FLAG_LIST_REFRESH = 16
 STAGES = ["sort", "predict", "nlist", "sweep", "finalize", "stab_sweep", "adaptive", "upsample",
           "upsample_sort", "mlp", "mlp_features", "mlp_fallback", "sweep_queue"]
 

@@ -15,6 +15,8 @@ constexpr int kScanTile = 2048;    // elements per block in the cell-table scan
 constexpr int kScanMaxParts = 8192;
 constexpr int kSmallCell = 32;     // cells up to this size are ranked one thread per entry
 constexpr int kBigLds = 4096;      // cells up to this size are sorted in LDS
+constexpr int kHugeChunks = 256;   // blocks compacting each larger cell
+constexpr int kHugeCounter = 63;   // ctr[63]: cells larger than kBigLds in this build
 constexpr double kPiD = 3.14159265358979323846;
 constexpr float kPiF = 3.14159265358979323846f;
 // A particle's list stays exact while its own accumulated correction and
@@ -31,6 +33,11 @@ constexpr float kListValidFrac = 0.9f;
 constexpr int kFallbackWaveMax = 32;
 constexpr int kFallbackWaveMaxLocal = 6;
 constexpr int kFallbackSlots = 62;  // ctr[1 + k % 62] = full-stencil count of sweep k
+// GG_FLAG_LIST_REFRESH: the lists are rebuilt after a refresh point when the
+// sweep's full-stencil count exceeds the first sweep's by more than
+// n / kRefreshFraction (a list build costs about as much as a stencil scan
+// of a few percent of the particles in each of the remaining sweeps)
+constexpr int kRefreshFraction = 32;
 // ctr[63..] are not used by the sweeps; the adaptive passes use ctr2 (own region)
 constexpr int kMergeRounds = 12;    // parallel claim-resolution rounds before the finisher
 

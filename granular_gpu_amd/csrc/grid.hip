@@ -25,7 +25,10 @@ __global__ void __launch_bounds__(kBlock) k_hash_count(const float *__restrict__
                                                        int *__restrict__ count,
                                                        int *__restrict__ ctr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) ctr[0] = 0;  // big-cell counter of this build
+  if (i == 0) {
+    ctr[0] = 0;                // big-cell counter of this build
+    ctr[kHugeCounter] = 0;     // cells for the multi-block path
+  }
   // All lanes take part in the wave operations; lanes past n carry key -1.
   int c = -1;
   if (i < n) {
@@ -201,17 +204,17 @@ __global__ void __launch_bounds__(kBlock) k_rank_small(const int *__restrict__ r
 
 // One block per queued cell (grid-stride over the queue).  Cells up to
 // kBigLds entries: bitonic sort of the old indices in LDS.  Larger cells
-// (e.g. the sentinel cell of particles outside the grid): stable
-// compaction of {i : key[i] == c} over the whole key array in index order,
-// one 1024-key tile at a time (ballot + block prefix sum), which is O(n)
-// and yields ascending old indices directly.
+// (e.g. the sentinel cell of particles outside the grid) go to the
+// multi-block path: the block records the cell and the range [lo, hi) of its
+// members' old indices in huge[] (k_huge_count / k_huge_write then compact
+// {i in [lo, hi) : key[i] == c} in index order over many blocks).
 __global__ void __launch_bounds__(kBlock) k_sort_big(const int *__restrict__ cell_start,
                                                      int *__restrict__ perm,
                                                      const int *__restrict__ big,
-                                                     const int *__restrict__ ctr,
-                                                     const int *__restrict__ key, int n) {
+                                                     int *__restrict__ ctr,
+                                                     int *__restrict__ huge) {
   __shared__ int buf[kBigLds];
-  __shared__ int wcnt[kBlock / 64];
+  __shared__ int red[2][kBlock / 64];
   const int nbig = ctr[0];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
@@ -242,40 +245,129 @@ __global__ void __launch_bounds__(kBlock) k_sort_big(const int *__restrict__ cel
       for (int k = threadIdx.x; k < len; k += blockDim.x) perm[s + k] = buf[k];
       __syncthreads();
     } else {
-      int base = 0;
-      for (int t0 = 0; t0 < n; t0 += 4 * kBlock) {
-        // thread owns keys t0 + 4*tid .. +3 (index order within the tile)
-        const int i0 = t0 + 4 * threadIdx.x;
-        int m[4];
-        int cnt = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          m[q] = (i0 + q < n && key[i0 + q] == c) ? 1 : 0;
-          cnt += m[q];
-        }
-        int inc = cnt;
-        for (int o = 1; o < 64; o <<= 1) {
-          const int v = __shfl_up(inc, o, 64);
-          if (lane >= o) inc += v;
-        }
-        if (lane == 63) wcnt[wid] = inc;
-        __syncthreads();
-        int woff = 0, tot = 0;
-        for (int w = 0; w < kBlock / 64; ++w) {
-          if (w < wid) woff += wcnt[w];
-          tot += wcnt[w];
-        }
-        int at = base + woff + inc - cnt;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (m[q]) {
-            if (at < len) perm[s + at] = i0 + q;
-            ++at;
-          }
-        }
-        base += tot;
-        __syncthreads();
+      int lo = 0x7fffffff, hi = -1;
+      for (int k = s + threadIdx.x; k < e; k += blockDim.x) {
+        const int v = perm[k];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
       }
+      for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_xor(lo, o, 64), z = __shfl_xor(hi, o, 64);
+        lo = a < lo ? a : lo;
+        hi = z > hi ? z : hi;
+      }
+      if (lane == 0) {
+        red[0][wid] = lo;
+        red[1][wid] = hi;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; ++w) {
+          lo = red[0][w] < lo ? red[0][w] : lo;
+          hi = red[1][w] > hi ? red[1][w] : hi;
+        }
+        const int h = atomicAdd(&ctr[kHugeCounter], 1);
+        huge[3 * h] = c;
+        huge[3 * h + 1] = lo;
+        huge[3 * h + 2] = hi + 1;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Chunk p of kHugeChunks equal parts of [lo, hi).
+__device__ __forceinline__ void huge_chunk(int lo, int hi, int p, int &a, int &b) {
+  const long long w = (long long)hi - lo;
+  a = lo + (int)(w * p / kHugeChunks);
+  b = lo + (int)(w * (p + 1) / kHugeChunks);
+}
+
+// Block p counts the members of each recorded cell in its chunk of the
+// cell's range: cnt[h * kHugeChunks + p].
+__global__ void __launch_bounds__(kBlock) k_huge_count(const int *__restrict__ key,
+                                                       const int *__restrict__ ctr,
+                                                       const int *__restrict__ huge,
+                                                       int *__restrict__ cnt) {
+  __shared__ int red[kBlock / 64];
+  const int nh = ctr[kHugeCounter];
+  const int p = blockIdx.x, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int h = 0; h < nh; ++h) {
+    const int c = huge[3 * h];
+    int a = 0, b = 0;
+    huge_chunk(huge[3 * h + 1], huge[3 * h + 2], p, a, b);
+    int m = 0;
+    for (int i = a + threadIdx.x; i < b; i += blockDim.x) m += key[i] == c ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) m += __shfl_xor(m, o, 64);
+    if (lane == 0) red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+      cnt[h * kHugeChunks + p] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Block p writes the members of each recorded cell found in its chunk, in
+// index order, after those of the chunks before it: a stable compaction of
+// {i in [lo, hi) : key[i] == c} into the cell's slots.
+__global__ void __launch_bounds__(kBlock) k_huge_write(const int *__restrict__ key,
+                                                       const int *__restrict__ cell_start,
+                                                       const int *__restrict__ ctr,
+                                                       const int *__restrict__ huge,
+                                                       const int *__restrict__ cnt,
+                                                       int *__restrict__ perm) {
+  __shared__ int wcnt[kBlock / 64];
+  const int nh = ctr[kHugeCounter];
+  const int p = blockIdx.x, lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int h = 0; h < nh; ++h) {
+    const int c = huge[3 * h];
+    const int s = cell_start[c], len = cell_start[c + 1] - s;
+    int a = 0, b = 0;
+    huge_chunk(huge[3 * h + 1], huge[3 * h + 2], p, a, b);
+    // members in the chunks before this one
+    int before = 0;
+    for (int q = threadIdx.x; q < p; q += blockDim.x) before += cnt[h * kHugeChunks + q];
+    for (int o = 32; o > 0; o >>= 1) before += __shfl_xor(before, o, 64);
+    if (lane == 0) wcnt[wid] = before;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < kBlock / 64; ++w) base += wcnt[w];
+    __syncthreads();
+    for (int t0 = a; t0 < b; t0 += 4 * kBlock) {
+      // thread owns keys t0 + 4*tid .. +3 (index order within the tile)
+      const int i0 = t0 + 4 * threadIdx.x;
+      int m[4];
+      int cn = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        m[q] = (i0 + q < b && key[i0 + q] == c) ? 1 : 0;
+        cn += m[q];
+      }
+      int inc = cn;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+      }
+      if (lane == 63) wcnt[wid] = inc;
+      __syncthreads();
+      int woff = 0, tot = 0;
+      for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) woff += wcnt[w];
+        tot += wcnt[w];
+      }
+      int at = base + woff + inc - cn;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (m[q]) {
+          if (at < len) perm[s + at] = i0 + q;
+          ++at;
+        }
+      }
+      base += tot;
+      __syncthreads();
     }
   }
 }
@@ -340,8 +432,19 @@ int sort_by_cell_keys(const float *pos, const int *keys_in, int n, int *cell_sta
     k_rank_small<<<blocks_for(n), kBlock, 0, st>>>(raw, n, key, cell_start, perm, big, ctr);
     GG_LAUNCH_CHECK();
   }
-  k_sort_big<<<256, kBlock, 0, st>>>(cell_start, perm, big, ctr, key, n);
+  // the huge-cell records and per-chunk counts live in the arrival-slot
+  // array, which nothing reads after the scatter: (3 + kHugeChunks) ints per
+  // cell of more than kBigLds particles (at most n / kBigLds of them)
+  int *huge = slot;
+  int *hcnt = slot + 3 * (n / kBigLds + 1);
+  k_sort_big<<<256, kBlock, 0, st>>>(cell_start, perm, big, ctr, huge);
   GG_LAUNCH_CHECK();
+  if (n > kBigLds) {
+    k_huge_count<<<kHugeChunks, kBlock, 0, st>>>(key, ctr, huge, hcnt);
+    GG_LAUNCH_CHECK();
+    k_huge_write<<<kHugeChunks, kBlock, 0, st>>>(key, cell_start, ctr, huge, hcnt, perm);
+    GG_LAUNCH_CHECK();
+  }
   return GG_OK;
 }
 

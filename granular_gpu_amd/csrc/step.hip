@@ -128,6 +128,18 @@ __global__ void __launch_bounds__(1024) k_reduce_bounds(const unsigned int *__re
   }
 }
 
+// Gate of a mid-step list refresh: the refresh runs when the full-stencil
+// count of the sweep just done (*ck) exceeds the first sweep's (*c0) by more
+// than threshold.  ck null: always run (the build of the step's lists).
+struct RefreshGate {
+  const int *__restrict__ ck;
+  const int *__restrict__ c0;
+  int threshold;
+};
+__device__ __forceinline__ bool gate_open(const RefreshGate &gt) {
+  return !gt.ck || *gt.ck - *gt.c0 > gt.threshold;
+}
+
 // Candidate lists: particles of the stencil of trunc(x*/L) with
 // |x*_i - x*_j| < r_i + r_j + skin (boundary: |x*_i - q| < R_b + max(r_i,
 // rmin) + skin), in scan order (cells ascending, index ascending).  x* is
@@ -140,7 +152,8 @@ __global__ void __launch_bounds__(BS)
             const float4 *__restrict__ bnd4, const int *__restrict__ cs_all,
             const int *__restrict__ bcs_all, DevGrid g0, SweepConst sc, float skin,
             int4 *__restrict__ meta, int *__restrict__ glist, int *__restrict__ blist,
-            int remap, SceneView sv) {
+            int remap, SceneView sv, RefreshGate gate) {
+  if (!gate_open(gate)) return;
   const int i = xcd_block(blockIdx.x, gridDim.x, remap) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const LocalGrid lg = local_grid<SCENES>(sv, i, g0, sc);
@@ -639,6 +652,38 @@ __global__ void __launch_bounds__(256) k_sweep_queue(SweepArgs a) {
   }
 }
 
+// Mid-step list refresh, first pass (when the gate is open): the current
+// predicted positions x* = base + offset into xs, every accumulated
+// displacement reset to zero, every displacement mark cleared, and the bound
+// on |x* - x0| (x0: the sorted start-of-step position, whose cell placed the
+// particle in the cell tables) raised to cover the new x*.  The list build
+// then runs on xs, so that each list again holds exactly the candidates
+// within the skin of x*, and the sweeps' validity tests restart from zero.
+__global__ void __launch_bounds__(kBlock)
+    k_refresh_prep(int n, const float4 *__restrict__ base, float4 *__restrict__ off,
+                   const float4 *__restrict__ x0, float4 *__restrict__ xs,
+                   int *__restrict__ dirty, int ndirty, unsigned int *__restrict__ bounds,
+                   RefreshGate gate) {
+  if (!gate_open(gate)) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = i; k < ndirty; k += gridDim.x * blockDim.x) dirty[k] = 0;
+  float dmax = 0.0f;
+  if (i < n) {
+    const float4 b = base[i];
+    const float4 o = off[i];
+    const float4 s0 = x0[i];
+    const float px = b.x + o.x, py = b.y + o.y, pz = b.z + o.z;
+    xs[i] = make_float4(px, py, pz, b.w);
+    off[i] = make_float4(o.x, o.y, o.z, 0.0f);
+    const float dx = px - s0.x, dy = py - s0.y, dz = pz - s0.z;
+    dmax = fmaxf(fabsf(dx), fmaxf(fabsf(dy), fabsf(dz)));
+    if (!(dmax == dmax)) dmax = __builtin_inff();
+  }
+  // one atomic per wave; non-negative floats order like their bits
+  const float wd = wave_max(dmax);
+  if ((threadIdx.x & 63) == 0) atomicMax(bounds, __float_as_uint(wd));
+}
+
 struct FinalConst {
   float inv_dt, sleep, damp_speed, damping, adaptive_damping;
 };
@@ -717,9 +762,9 @@ template <int BS, bool SCENES>
 static int launch_nlist_bs(int n, const float4 *x, const float4 *xs, const float4 *bnd4,
                            const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
                            int4 *meta, int *glist, int *blist, int remap, SceneView sv,
-                           hipStream_t st) {
+                           RefreshGate gate, hipStream_t st) {
   k_nlist<BS, SCENES><<<(n + BS - 1) / BS, BS, 0, st>>>(n, x, xs, bnd4, cs, bcs, g, sc, skin,
-                                                         meta, glist, blist, remap, sv);
+                                                         meta, glist, blist, remap, sv, gate);
   GG_LAUNCH_CHECK();
   return GG_OK;
 }
@@ -728,24 +773,24 @@ template <bool SCENES>
 static int launch_nlist_sc(int bs, int n, const float4 *x, const float4 *xs, const float4 *bnd4,
                            const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
                            int4 *meta, int *glist, int *blist, int remap, SceneView sv,
-                           hipStream_t st) {
+                           RefreshGate gate, hipStream_t st) {
   switch (bs) {
-    case 64: return launch_nlist_bs<64, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, st);
-    case 128: return launch_nlist_bs<128, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, st);
-    case 512: return launch_nlist_bs<512, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, st);
-    default: return launch_nlist_bs<256, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, st);
+    case 64: return launch_nlist_bs<64, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, gate, st);
+    case 128: return launch_nlist_bs<128, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, gate, st);
+    case 512: return launch_nlist_bs<512, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, gate, st);
+    default: return launch_nlist_bs<256, SCENES>(n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist, remap, sv, gate, st);
   }
 }
 
 static int launch_nlist(int bs, int n, const float4 *x, const float4 *xs, const float4 *bnd4,
                         const int *cs, const int *bcs, DevGrid g, SweepConst sc, float skin,
                         int4 *meta, int *glist, int *blist, int remap, const SceneView *sv,
-                        hipStream_t st) {
+                        hipStream_t st, RefreshGate gate = RefreshGate{nullptr, nullptr, 0}) {
   if (sv)
     return launch_nlist_sc<true>(bs, n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist,
-                                 remap, *sv, st);
+                                 remap, *sv, gate, st);
   return launch_nlist_sc<false>(bs, n, x, xs, bnd4, cs, bcs, g, sc, skin, meta, glist, blist,
-                                remap, SceneView{nullptr, nullptr, 0}, st);
+                                remap, SceneView{nullptr, nullptr, 0}, gate, st);
 }
 
 // permute (perm may be null = identity) + predict + lists.  sv: scenes
@@ -890,6 +935,47 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   return GG_OK;
 }
 
+// Whether a list refresh may follow sweep k: after the last stabilisation
+// sweep (where the position base changes) and then after every fourth solver
+// sweep, never after the last sweep.
+static bool refresh_point(const gg_params *p, const gg_workspace *ws, int k) {
+  if (!(p->flags & GG_FLAG_LIST_REFRESH) || !lists_on(p, ws)) return false;
+  const int S = p->stab_iters, K = p->stab_iters + p->solver_iters;
+  if (S == 0 || k < S - 1 || k >= K - 1) return false;
+  return (k - (S - 1)) % 4 == 0;
+}
+
+// The refresh after sweep k (gated on the device by the sweep's full-stencil
+// count): positions, marks and bounds by k_refresh_prep, then the list build
+// on the refreshed positions.  Sweeps after the last stabilisation sweep no
+// longer read d0, which holds the refreshed positions.
+static int run_refresh(const gg_particles *set, const int *cs, const int *bcs,
+                       const gg_params *p, int k, const gg_workspace *ws, hipStream_t st,
+                       const SceneView *sv = nullptr) {
+  Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
+                         ws->bmax);
+  const int n = set->n;
+  if (n <= 0) return GG_OK;
+  const int *ctr = at<int>(ws, L.ctr) + 1;
+  const RefreshGate gate{ctr + (k % kFallbackSlots), ctr, n / kRefreshFraction};
+  // the offsets the next sweep reads and its position base
+  float4 *off = ((k + 1) % 2 == 0) ? at<float4>(ws, L.off0) : at<float4>(ws, L.off1);
+  const float4 *base = at<float4>(ws, L.xb);
+  float4 *xs = at<float4>(ws, L.d0);
+  const int ndirty = (int)(grid_cells(ws->grid) + 1);
+  prof_begin(GG_STAGE_NLIST, st);
+  k_refresh_prep<<<blocks_for(n), kBlock, 0, st>>>(n, base, off, at<float4>(ws, L.x0), xs,
+                                                    at<int>(ws, L.dirty), ndirty,
+                                                    at<unsigned int>(ws, L.bounds), gate);
+  GG_LAUNCH_CHECK();
+  GG_TRY(launch_nlist(p->nlist_block == 0 ? 64 : p->nlist_block, n, at<float4>(ws, L.x0), xs,
+                      at<float4>(ws, L.bnd4), cs, bcs, dev_grid(ws->grid), sweep_const(p, ws),
+                      p->skin, at<int4>(ws, L.meta), at<int>(ws, L.glist), at<int>(ws, L.blist),
+                      (p->flags & GG_FLAG_NO_XCD_REMAP) ? 0 : 1, sv, st, gate));
+  prof_end(GG_STAGE_NLIST, st);
+  return GG_OK;
+}
+
 static int run_finalize(gg_particles *set, gg_adaptive *ad, const gg_params *p,
                         const gg_workspace *ws, hipStream_t st) {
   Layout L = make_layout(ws->capacity, ws->boundary_capacity, ws->grid, ws->max_sweeps, ws->gmax,
@@ -951,8 +1037,11 @@ extern "C" int gg_step(gg_particles *set, gg_adaptive *ad, int *cell_start,
   GG_TRY(run_predict(set, ad, at<int>(ws, L.perm), at<int>(ws, L.key), cell_start, bcell_start,
                      boundary->n, params, ws, stream));
   const int K = params->stab_iters + params->solver_iters;
-  for (int k = 0; k < K; ++k)
+  for (int k = 0; k < K; ++k) {
     GG_TRY(run_sweep(set, cell_start, bcell_start, params, k, ws, stream));
+    if (refresh_point(params, ws, k))
+      GG_TRY(run_refresh(set, cell_start, bcell_start, params, k, ws, stream));
+  }
   return run_finalize(set, ad, params, ws, stream);
 }
 
@@ -980,9 +1069,12 @@ extern "C" int gg_step_scenes(gg_particles *set, int *cell_start, const gg_parti
   GG_TRY(run_predict(set, nullptr, at<int>(ws, L.perm), at<int>(ws, L.key), cell_start,
                      bcell_start, boundary->n, params, ws, stream, &sv));
   const int K = params->stab_iters + params->solver_iters;
-  for (int k = 0; k < K; ++k)
+  for (int k = 0; k < K; ++k) {
     GG_TRY(run_sweep(set, cell_start, bcell_start, params, k, ws, stream, &sv,
                      scenes->friction != 0));
+    if (refresh_point(params, ws, k))
+      GG_TRY(run_refresh(set, cell_start, bcell_start, params, k, ws, stream, &sv));
+  }
   return run_finalize(set, nullptr, params, ws, stream);
 }
 

@@ -102,6 +102,15 @@ typedef struct gg_params {
  * sweep.  Results are the same bit for bit. */
 #define GG_FLAG_FALLBACK_LANE 4
 #define GG_FLAG_FALLBACK_WAVE 8
+/* Lists that lapse inside a step (particles driven far by their contacts,
+ * e.g. columns pushed through their floors) are rebuilt from the current
+ * positions after the last stabilisation sweep and then after every fourth
+ * solver sweep, when that sweep scanned the stencil for more than n/32
+ * particles beyond the first sweep's count (decided on the device, so the
+ * step stays one fixed sequence of launches).  Results are the same bit for
+ * bit; each refresh point adds two launches, which exit at once when the
+ * lists still hold. */
+#define GG_FLAG_LIST_REFRESH 16
 
 /* Device arrays of one particle set.  Mirrors GranularParticles
  * (GranularParticles.hpp:5-110) + Particles (Particles.hpp:7-62).

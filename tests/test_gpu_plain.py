@@ -33,7 +33,7 @@ def test_device_is_gfx950(gpu):
     assert gpu.gg_device_check() == 0
 
 
-@pytest.mark.parametrize("case", ["lattice", "random_outside", "dense_cells"])
+@pytest.mark.parametrize("case", ["lattice", "random_outside", "dense_cells", "huge_cells"])
 def test_grid_build_bitexact(gpu, case):
     import torch
     _log(f"grid_build case={case}")
@@ -51,6 +51,18 @@ def test_grid_build_bitexact(gpu, case):
             np.float32)
         c = rng.uniform(0.2, 0.3, size=(3000, 3)).astype(np.float32)
         pos = np.concatenate([a, b, c])[rng.permutation(10000)]
+    elif case == "huge_cells":
+        # three cells of more than 4096 particles (one the sentinel) whose
+        # members sit in different index ranges, with scattered strays
+        a = np.full((6000, 3), 0.41, np.float32) + rng.uniform(0, 0.005, (6000, 3)).astype(
+            np.float32)
+        b = np.full((9000, 3), 0.77, np.float32) + rng.uniform(0, 0.005, (9000, 3)).astype(
+            np.float32)
+        out = rng.uniform(1.2, 1.5, size=(8000, 3)).astype(np.float32)
+        rest = rng.uniform(0.0, 1.0, size=(7000, 3)).astype(np.float32)
+        pos = np.concatenate([rest[:2000], a, rest[2000:4000], b, out, rest[4000:]])
+        swap = rng.choice(pos.shape[0], size=600, replace=False)
+        pos[swap] = pos[swap[::-1]]
     else:
         pos = s.pos
     s.pos = pos
