@@ -340,8 +340,12 @@ def batch_phase(args, lib, torch, C, np):
                      for sc, c in zip(scs, cells))
     print(f"# batch: {len(scs)} scenes, {total} particles", file=sys.stderr, flush=True)
 
+    # lists lapse inside the step once the columns break through their
+    # floors at this time step: rebuilt mid-step when the sweeps' stencil
+    # scans say so (gated on the device, results unchanged)
+    from granular_gpu_amd._lib import GG_FLAG_LIST_REFRESH
     # one set, one launch per stage
-    sb = SceneBatch(scs, fr, stab=3, solver=10)
+    sb = SceneBatch(scs, fr, stab=3, solver=10, flags=GG_FLAG_LIST_REFRESH)
     sb.step(args.batch_warmup)
     sb.capture(steps_per_launch=1)
     tune1 = sb.autotune(trials=5)
@@ -358,7 +362,7 @@ def batch_phase(args, lib, torch, C, np):
     # each scene's own step chain in one graph, and eagerly, over the same
     # steps from fresh copies of the scenes
     def per_scene(eager_mode):
-        sims = [PlainSim(sc, stab=3, solver=10, mu_s=ms, mu_k=mk)
+        sims = [PlainSim(sc, stab=3, solver=10, mu_s=ms, mu_k=mk, flags=GG_FLAG_LIST_REFRESH)
                 for sc, (ms, mk) in zip(scs, fr)]
         b = BatchSim(sims, branches=min(4, len(sims)))
         b.step(args.batch_warmup)
@@ -385,6 +389,7 @@ def batch_phase(args, lib, torch, C, np):
     return {
         "workload": f"{len(scs)} plain box scenes, counts log-uniform in [4k, 256k] (seed 4), "
                     f"friction mu_s U(0.3, 0.7), mu_k = 0.8 mu_s, 3 + 10 sweeps, dt = {dt:.6g}, "
+                    f"mid-step list refresh on, "
                     f"steps {args.batch_warmup}..{args.batch_warmup + args.batch_steps}",
         "scenes": len(scs), "particles": total,
         "particle_steps_per_s": round(total / fused, 1),

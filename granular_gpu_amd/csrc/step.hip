@@ -679,9 +679,19 @@ __global__ void __launch_bounds__(kBlock)
     dmax = fmaxf(fabsf(dx), fmaxf(fabsf(dy), fabsf(dz)));
     if (!(dmax == dmax)) dmax = __builtin_inff();
   }
-  // one atomic per wave; non-negative floats order like their bits
+  // the block's maximum, then one atomic only where it raises the bound
+  // (non-negative floats order like their bits; most blocks stay below the
+  // bound of the step's start, so the atomics do not pile up on one address)
+  __shared__ float red[kBlock / 64];
   const float wd = wave_max(dmax);
-  if ((threadIdx.x & 63) == 0) atomicMax(bounds, __float_as_uint(wd));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wd;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float bd = red[0];
+    for (int w = 1; w < kBlock / 64; ++w) bd = fmaxf(bd, red[w]);
+    const unsigned int bits = __float_as_uint(bd);
+    if (bits > __atomic_load_n(bounds, __ATOMIC_RELAXED)) atomicMax(bounds, bits);
+  }
 }
 
 struct FinalConst {

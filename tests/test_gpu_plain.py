@@ -188,6 +188,39 @@ def test_fallback_paths_agree(gpu):
     assert np.array_equal(runs["wave"][0][1], ov)
 
 
+def test_list_refresh_matches_oracle(gpu):
+    # C1 with friction at dt 1/240: around step 100 the column is pushed
+    # through the floor and lists lapse inside the step; with
+    # GG_FLAG_LIST_REFRESH they are rebuilt after the refresh points when the
+    # sweep scanned the stencil for n/32 more particles than the first sweep
+    # (sweeps 2, 6, 10 of 3 + 10).  The result must not change: the
+    # refreshed run equals the run without refresh and the oracle bit for
+    # bit, and the refresh must have happened (a sweep right after a refresh
+    # point scans fewer stencils than the one before it, by more than n/32).
+    import torch
+    from granular_gpu_amd import _lib
+    steps = 110
+    s = scenes.plain_box(32, 1.0)
+    n = s.n
+    runs, drop = {}, 0
+    for name, flags in (("plain", 0), ("refresh", _lib.GG_FLAG_LIST_REFRESH)):
+        sim = _sim(s, stab=3, solver=10, mu_s=0.5, mu_k=0.4, flags=flags)
+        for _ in range(steps):
+            sim.step(1)
+            if flags:
+                c = sim.view(8, 64, dtype=torch.int32, width=1).reshape(-1).cpu().numpy()[1:14]
+                drop = max(drop, max(int(c[k]) - int(c[k + 1]) for k in (2, 6, 10)))
+        runs[name] = sim.host()
+    _log(f"list refresh, C1 + friction, {steps} steps: largest drop after a refresh point "
+         f"{drop} stencil scans (n/32 = {n // 32})")
+    assert drop > n // 32
+    for x, y in zip(runs["refresh"], runs["plain"]):
+        assert np.array_equal(x, y)
+    op, ov, _, _, _, _ = _oracle_steps(s, steps, 3, 10, 0.5, 0.4)
+    assert np.array_equal(runs["refresh"][0], op)
+    assert np.array_equal(runs["refresh"][1], ov)
+
+
 def test_step_graph_replays_equal_eager_steps(gpu):
     # every replay of the captured step (eager work and a synchronisation
     # between replays) must equal the same steps launched kernel by kernel
