@@ -33,11 +33,11 @@ constexpr float kListValidFrac = 0.9f;
 constexpr int kFallbackWaveMax = 32;
 constexpr int kFallbackWaveMaxLocal = 6;
 constexpr int kFallbackSlots = 62;  // ctr[1 + k % 62] = full-stencil count of sweep k
-// GG_FLAG_LIST_REFRESH: the lists are rebuilt after a refresh point when the
-// sweep's full-stencil count exceeds the first sweep's by more than
-// n / kRefreshFraction (a list build costs about as much as a stencil scan
-// of a few percent of the particles in each of the remaining sweeps)
-constexpr int kRefreshFraction = 32;
+// GG_FLAG_LIST_REFRESH: a list build costs about as much as kRefreshCost
+// stencil scans per particle rebuilt, measured: a scan by a lane whose wave
+// holds lists is divergent and waits on its dependent loads (about 1.3 ns per
+// scan against 0.15 ns per particle of a build at 4M particles)
+constexpr int kRefreshCost = 8;
 // ctr[63..] are not used by the sweeps; the adaptive passes use ctr2 (own region)
 constexpr int kMergeRounds = 12;    // parallel claim-resolution rounds before the finisher
 

@@ -945,20 +945,23 @@ static int run_sweep(const gg_particles *set, const int *cs, const int *bcs, con
   return GG_OK;
 }
 
-// Whether a list refresh may follow sweep k: after the last stabilisation
-// sweep (where the position base changes) and then after every fourth solver
-// sweep, never after the last sweep.
+// Whether a list refresh may follow sweep k: after every sweep but the first
+// and the last (the gate on the device decides whether it runs).
 static bool refresh_point(const gg_params *p, const gg_workspace *ws, int k) {
   if (!(p->flags & GG_FLAG_LIST_REFRESH) || !lists_on(p, ws)) return false;
-  const int S = p->stab_iters, K = p->stab_iters + p->solver_iters;
-  if (S == 0 || k < S - 1 || k >= K - 1) return false;
-  return (k - (S - 1)) % 4 == 0;
+  const int K = p->stab_iters + p->solver_iters;
+  return k >= 1 && k <= K - 2;
 }
 
-// The refresh after sweep k (gated on the device by the sweep's full-stencil
-// count): positions, marks and bounds by k_refresh_prep, then the list build
-// on the refreshed positions.  Sweeps after the last stabilisation sweep no
-// longer read d0, which holds the refreshed positions.
+// The refresh after sweep k, gated on the device: it runs when sweep k
+// scanned the stencil for more particles than sweep 0 (whose lists were
+// fresh) by more than n / (kRefreshCost * R), R = the sweeps left in the
+// step, i.e. when the scans it can save over the rest of the step outweigh
+// one list build.  Positions, marks and bounds by k_refresh_prep, then the
+// list build on the refreshed positions.  The positions go to a buffer that
+// no later sweep of the step reads before rewriting it: the stabilisation
+// base (written by the last stabilisation sweep) while stabilisation
+// sweeps remain, dt*v (read only by the last stabilisation sweep) after.
 static int run_refresh(const gg_particles *set, const int *cs, const int *bcs,
                        const gg_params *p, int k, const gg_workspace *ws, hipStream_t st,
                        const SceneView *sv = nullptr) {
@@ -966,12 +969,16 @@ static int run_refresh(const gg_particles *set, const int *cs, const int *bcs,
                          ws->bmax);
   const int n = set->n;
   if (n <= 0) return GG_OK;
+  const int S = p->stab_iters, K = p->stab_iters + p->solver_iters;
+  const int left = K - 1 - k;
   const int *ctr = at<int>(ws, L.ctr) + 1;
-  const RefreshGate gate{ctr + (k % kFallbackSlots), ctr, n / kRefreshFraction};
+  const RefreshGate gate{ctr + (k % kFallbackSlots), ctr,
+                         (int)((long long)n / ((long long)kRefreshCost * left))};
   // the offsets the next sweep reads and its position base
   float4 *off = ((k + 1) % 2 == 0) ? at<float4>(ws, L.off0) : at<float4>(ws, L.off1);
-  const float4 *base = at<float4>(ws, L.xb);
-  float4 *xs = at<float4>(ws, L.d0);
+  const bool stab_next = k + 1 < S;
+  const float4 *base = (S == 0 || stab_next) ? at<float4>(ws, L.x0) : at<float4>(ws, L.xb);
+  float4 *xs = stab_next ? at<float4>(ws, L.xb) : at<float4>(ws, L.d0);
   const int ndirty = (int)(grid_cells(ws->grid) + 1);
   prof_begin(GG_STAGE_NLIST, st);
   k_refresh_prep<<<blocks_for(n), kBlock, 0, st>>>(n, base, off, at<float4>(ws, L.x0), xs,

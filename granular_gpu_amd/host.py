@@ -84,6 +84,7 @@ def load() -> C.CDLL:
         "ggh_system_translate_animated": (C.c_int, [vp, C.c_float, C.c_float, C.c_float]),
         "ggh_system_frame_times": (C.c_int, [vp, fp, C.c_int]),
         "ggh_system_particle_stats": (C.c_int, [vp, ip, C.c_int]),
+        "ggh_system_sweep_scans": (C.c_int, [vp, ip, C.c_int]),
         "ggh_system_save_stats": (C.c_int, [vp, C.c_char_p]),
         "ggh_system_set_solver": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float,
                                             C.c_int]),
@@ -355,6 +356,16 @@ class HostSystem:
         n = self.lib.ggh_system_particle_stats(self.h, None, 0)
         out = np.zeros((n, 3), np.int32)
         self.lib.ggh_system_particle_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int)), n)
+        return out
+
+    def sweep_scans(self) -> np.ndarray:
+        """Particles that scanned their whole stencil in each sweep of the
+        last step."""
+        k = self.lib.ggh_system_sweep_scans(self.h, None, 0)
+        out = np.zeros(max(k, 0), np.int32)
+        rc = self.lib.ggh_system_sweep_scans(self.h, out.ctypes.data_as(C.POINTER(C.c_int)), k)
+        if rc < 0:
+            _chk(self.lib, rc, "ggh_system_sweep_scans")
         return out
 
     def save_stats(self, path):

@@ -12,6 +12,9 @@ void check(int rc, const char *what) {
     throw std::runtime_error(std::string(what) + ": " + gg_status_string(rc));
 }
 
+// gg_workspace_view(8): counter 1 + k % 62 holds sweep k's stencil-scan count
+constexpr int kScanSlots = 62;
+
 }  // namespace
 
 Solver::Solver(const std::shared_ptr<GranularParticles> &particles)
@@ -367,6 +370,21 @@ void Solver::update_neighborhood(const std::shared_ptr<GranularParticles> &parti
   gg_particles s = particles->view();
   check(gg_sort_positions_by_key(&s, _key_cells->addr(), &_ws, nullptr),
         "gg_sort_positions_by_key");
+}
+
+int Solver::sweep_scan_counts(int *out, int max) const {
+  const int sweeps = _stab + _solver_iters;
+  if (!_ws_ready || !out || max <= 0) return sweeps;
+  const int *ctr = static_cast<const int *>(gg_workspace_view(&_ws, 8));
+  if (!ctr) throw std::runtime_error("sweep_scan_counts: no workspace");
+  const int k = sweeps < max ? sweeps : max;
+  // ctr[1 + k] holds sweep k's count (sweeps beyond the slot count wrap)
+  const int m = k < kScanSlots ? k : kScanSlots;
+  if (hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(out, ctr + 1, sizeof(int) * (size_t)m, hipMemcpyDeviceToHost) != hipSuccess)
+    throw std::runtime_error("sweep_scan_counts: copy failed");
+  for (int i = m; i < k; ++i) out[i] = out[i % kScanSlots];
+  return sweeps;
 }
 
 void Solver::save_particle_stats(const std::string &filename) {

@@ -452,6 +452,15 @@ int ggh_system_particle_stats(const ggh_system *h, int *out, int max_rows) {
   return (int)st.size();
 }
 
+int ggh_system_sweep_scans(const ggh_system *h, int *out, int max) {
+  try {
+    if (!h) throw std::invalid_argument("bad arguments");
+    return h->sys->get_solver().sweep_scan_counts(out, max);
+  } catch (const std::exception &e) {
+    return fail(e);
+  }
+}
+
 int ggh_system_save_stats(ggh_system *h, const char *path) {
   try {
     if (!h || !path) throw std::invalid_argument("bad arguments");

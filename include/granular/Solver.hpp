@@ -124,6 +124,11 @@ class Solver {
   float3 get_explosion_center() const { return _explosion_center; }
 
   void save_particle_stats(const std::string &filename);
+  // Particles that scanned their whole 27-cell stencil (instead of their
+  // neighbour list) in each sweep of the last step, sweep 0 first (at most
+  // max counts; returns the number of sweeps).  A diagnostic of the lists:
+  // results never depend on it.
+  int sweep_scan_counts(int *out, int max) const;
   const std::vector<std::tuple<int, int, int>> &get_particle_stats() const {
     return _particle_stats;
   }

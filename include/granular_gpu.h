@@ -104,12 +104,12 @@ typedef struct gg_params {
 #define GG_FLAG_FALLBACK_WAVE 8
 /* Lists that lapse inside a step (particles driven far by their contacts,
  * e.g. columns pushed through their floors) are rebuilt from the current
- * positions after the last stabilisation sweep and then after every fourth
- * solver sweep, when that sweep scanned the stencil for more than n/32
- * particles beyond the first sweep's count (decided on the device, so the
- * step stays one fixed sequence of launches).  Results are the same bit for
- * bit; each refresh point adds two launches, which exit at once when the
- * lists still hold. */
+ * positions after a sweep (any but the first and the last) that scanned the
+ * stencil for more particles than the first sweep by more than n/(8 R), R
+ * the sweeps left in the step: when the scans saved outweigh one list
+ * build (decided on the device, so the step stays one fixed sequence of
+ * launches).  Results are the same bit for bit; each refresh point adds two
+ * launches, which exit at once when the lists still hold. */
 #define GG_FLAG_LIST_REFRESH 16
 
 /* Device arrays of one particle set.  Mirrors GranularParticles

@@ -123,6 +123,10 @@ int ggh_system_move_picked(ggh_system *h, const int *indices, int count, const f
 int ggh_system_frame_times(const ggh_system *h, float *out, int max);
 /* Rows (merged, split, N before) of the adaptive passes; returns the count. */
 int ggh_system_particle_stats(const ggh_system *h, int *out, int max_rows);
+/* Solver::sweep_scan_counts: particles that scanned their whole stencil in
+ * each sweep of the last step (up to max); returns the sweep count, -1 on
+ * error.  No counterpart in the reference (a diagnostic of the lists). */
+int ggh_system_sweep_scans(const ggh_system *h, int *out, int max);
 int ggh_system_save_stats(ggh_system *h, const char *path);
 
 /* Output formats (include/granular/Output.hpp).  VTK: the reference's

@@ -54,8 +54,13 @@ def test_excavator_4m_bucket_200k_adaptive_learned_32m(gpu):
                                       boundary_animated=dig.animated)
     sysm.set_merge_rules()  # radius levels r/2, r, 2r: masses 0.125, 1, 8
     # the bucket's 200k particles are 3.4 mm apart: a particle beside it has
-    # about 80 boundary candidates within contact distance plus the skin
-    sysm.set_lists(gmax=48, bmax=96, skin=0.75 * R)
+    # about 80 boundary candidates within contact distance plus the skin;
+    # merged (2r) particles among split (r/2) ones have 100+ granular ones
+    sysm.set_lists(gmax=160, bmax=96, skin=0.75 * R)
+    # lists lapse inside the step where splits overlap and the bucket pushes:
+    # rebuilt mid-step when the sweeps' stencil scans say so
+    from granular_gpu_amd import _lib
+    sysm.set_solver(3, 5, 0.0, 0.0, _lib.GG_FLAG_LIST_REFRESH)
     # the network's guards in units of the coarse radius as at r = 0.02
     # (boundary 2.5 r, floor 4 r, sides 2.5 r)
     sysm.set_learned(w, b, radius=R, guard_boundary=2.5 * R, guard_floor=4 * R,
