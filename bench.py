@@ -11,8 +11,10 @@ neighbour lists + 13 contact sweeps + finalize, inputs resident in HBM.
 
 Before any warm-up, the scene is advanced --settle steps (default 100)
 outside every timed region: the lattice starts 0.1 m above the floor and
-lands around step 71, so the timed window (steps settle+warmup ..
-settle+warmup+steps) measures the settled pile, not the free fall.
+its bottom layers land around step 71, so the timed window (steps
+settle+warmup .. settle+warmup+steps) measures the landed column (lower
+layers on the floor, the upper ones still falling onto them: median speed
+~1.8 m/s at steps 100..125), not the untouched free fall.
 
 Prints one JSON line.  `value` = particle-steps/s over the timed steps.
 Per-stage times come from hipEvents on the launch stream around every stage
@@ -362,7 +364,9 @@ def batch_phase(args, lib, torch, C, np):
     # each scene's own step chain in one graph, and eagerly, over the same
     # steps from fresh copies of the scenes
     def per_scene(eager_mode):
-        sims = [PlainSim(sc, stab=3, solver=10, mu_s=ms, mu_k=mk, flags=GG_FLAG_LIST_REFRESH)
+        # (without the refresh: its two launches per refresh point would be
+        # paid 64 times per step here)
+        sims = [PlainSim(sc, stab=3, solver=10, mu_s=ms, mu_k=mk)
                 for sc, (ms, mk) in zip(scs, fr)]
         b = BatchSim(sims, branches=min(4, len(sims)))
         b.step(args.batch_warmup)
@@ -380,16 +384,20 @@ def batch_phase(args, lib, torch, C, np):
         torch.cuda.empty_cache()
         return el, tune
 
-    graph, tune2 = per_scene(False)
-    eager, _ = per_scene(True)
+    if args.batch_one_set:
+        graph = eager = float("nan")
+        tune2 = None
+    else:
+        graph, tune2 = per_scene(False)
+        eager, _ = per_scene(True)
     print(f"# batch timed {args.batch_steps} launches: one set {1000 * fused:.3f} ms/step, "
           f"per-scene graph {1000 * graph:.3f}, eager {1000 * eager:.3f}", file=sys.stderr,
           flush=True)
     ach = bytes_step / fused / 1e9
-    return {
+    out = {
         "workload": f"{len(scs)} plain box scenes, counts log-uniform in [4k, 256k] (seed 4), "
                     f"friction mu_s U(0.3, 0.7), mu_k = 0.8 mu_s, 3 + 10 sweeps, dt = {dt:.6g}, "
-                    f"mid-step list refresh on, "
+                    f"mid-step list refresh on (one set), "
                     f"steps {args.batch_warmup}..{args.batch_warmup + args.batch_steps}",
         "scenes": len(scs), "particles": total,
         "particle_steps_per_s": round(total / fused, 1),
@@ -403,6 +411,10 @@ def batch_phase(args, lib, torch, C, np):
         "step_hbm_model": {"bytes_per_step": int(bytes_step), "achieved_GBs": round(ach, 1),
                            "frac_of_hbm_roofline": round(ach / HBM_PEAK_GBS, 4)},
     }
+    if args.batch_one_set:
+        for key in ("per_scene_graph", "eager_particle_steps_per_s", "eager_ms_per_step"):
+            out.pop(key)
+    return out
 
 
 def main():
@@ -439,6 +451,9 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-warmup", type=int, default=20)
+    ap.add_argument("--batch-one-set", action="store_true",
+                    help="time only the one-set batch (no per-scene graph / eager runs; "
+                         "profiling)")
     ap.add_argument("--batch-dt", type=float, default=1.0 / 240.0,
                     help="time step of the batched scenes (the plain box's 1/240)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -649,7 +664,7 @@ def main():
             "workload": f"plain box {args.side}^3 jittered lattice (r=0.01, seed 1) in a "
                         f"{args.box}^3 m box, {scene.cells[0]}^3 cells, {nb} wall particles, "
                         f"dt={args.dt}, g=-9.8, {S} stabilisation + {I} solver sweeps, "
-                        f"friction {mu_s}/{mu_k}, FP32, settled pile: steps {t_first}.."
+                        f"friction {mu_s}/{mu_k}, FP32, landed column: steps {t_first}.."
                         f"{t_first + args.steps} (after {args.settle} settling + "
                         f"{args.warmup} warm-up steps), neighbour-list skin {sim.params.skin}",
             "particles": n, "boundary_particles": nb, "cells": cells,
