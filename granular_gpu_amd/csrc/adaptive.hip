@@ -53,6 +53,13 @@ __device__ __forceinline__ float3 nrm(float x, float y, float z) {
 }
 
 // f(j) for every particle j of the stencil of trunc(p/L), in scan order.
+// False only when fl(sqrt(d2)) <= lim is impossible (the prefilter of the
+// contact terms, whose bound holds while lim^2 is a normal float; below
+// that, and for NaN, the exact test decides alone).
+__device__ __forceinline__ bool maybe_within(float d2, float lim) {
+  return !(lim >= 0x1p-60f) || d2 <= contact_prefilter(lim);
+}
+
 template <class F>
 __device__ __forceinline__ void for_stencil(float px, float py, float pz, const DevGrid &g,
                                             const int *__restrict__ cs, F &&f) {
@@ -85,8 +92,12 @@ __global__ void __launch_bounds__(kBlock)
   for_stencil(px, py, pz, g, cs, [&](int j) {
     if (j == i) return;
     const float bx = px - pos[3 * j], by = py - pos[3 * j + 1], bz = pz - pos[3 * j + 2];
-    const float dis = sqrtf(bx * bx + by * by + bz * bz);
-    if (!(dis <= 2.0f * (ri + rs[j]))) return;
+    const float d2 = bx * bx + by * by + bz * bz;
+    const float lim = 2.0f * (ri + rs[j]);
+    // (a candidate failing the prefilter fails the test: no square root)
+    if (!maybe_within(d2, lim)) return;
+    const float dis = sqrtf(d2);
+    if (!(dis <= lim)) return;
     const float3 u = nrm(bx, by, bz);
     cx = cx + u.x;
     cy = cy + u.y;
@@ -112,8 +123,11 @@ __global__ void __launch_bounds__(kBlock)
   for_stencil(px, py, pz, g, cs, [&](int j) {
     if (j == i) return;
     const float bx = pos[3 * j] - px, by = pos[3 * j + 1] - py, bz = pos[3 * j + 2] - pz;
-    const float dis = sqrtf(bx * bx + by * by + bz * bz);
-    if (!(dis <= 2.0f * (ri + rs[j]))) return;
+    const float d2 = bx * bx + by * by + bz * bz;
+    const float lim = 2.0f * (ri + rs[j]);
+    if (!maybe_within(d2, lim)) return;
+    const float dis = sqrtf(d2);
+    if (!(dis <= lim)) return;
     const float3 u = nrm(bx, by, bz);
     if (u.x * cv.x + u.y * cv.y + u.z * cv.z >= 0.5f) vote = true;
     ++nn;
@@ -203,8 +217,12 @@ __global__ void __launch_bounds__(kBlock)
   for_stencil(px, py, pz, g, cs, [&](int j) {
     if (j >= n) return;
     const float dx = px - pos[3 * j], dy = py - pos[3 * j + 1], dz = pz - pos[3 * j + 2];
-    const float dis = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float d2 = dx * dx + dy * dy + dz * dz;
     const float lim = partner_dist(R, ri, R.depth ? rs[j] : 0.0f);
+    // everything below needs dis < lim, which a candidate failing the
+    // prefilter cannot meet
+    if (!maybe_within(d2, lim)) return;
+    const float dis = sqrtf(d2);
     if (dis < lim) {
       ld = ld + mass[j];
       ++lc;
@@ -432,7 +450,8 @@ __global__ void __launch_bounds__(kBlock)
           for (int k = bcs[cell]; k < bcs[cell + 1]; ++k) {
             const float4 q = bnd4[k];
             const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
-            if (sqrtf(dx * dx + dy * dy + dz * dz) < maxd) {
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (maybe_within(d2, maxd) && sqrtf(d2) < maxd) {
               blocked = true;
               break;
             }
