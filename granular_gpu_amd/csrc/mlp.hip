@@ -309,38 +309,50 @@ __global__ void __launch_bounds__(256)
   const Packed3 vi = ld3(vel, i);
   const float vx = vi.x, vy = vi.y, vz = vi.z;
   const float inv_r = 1.0f / c.radius, inv_v = 1.0f / c.vscale;
-  // two neighbour slots = 12 features = 3 groups of 4
+  // Neighbours in groups of kGather: a group's positions and velocities are
+  // loaded together (an empty slot loads the particle itself and writes
+  // zeros), so that the features wait for kKnn / kGather rounds of loads,
+  // not 2 kKnn.
+  constexpr int kGather = 4;
 #pragma unroll
-  for (int s = 0; s < kKnn; s += 2) {
-    float f[12];
+  for (int h = 0; h < kKnn; h += kGather) {
+    Packed3 pj[kGather], vj[kGather];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int j = kj[s + u];
-      if (knn) knn[(size_t)i * kKnn + s + u] = j;
-      if (j >= 0) {
-        const Packed3 pj = ld3(pos, j), vj = ld3(vel, j);
-        f[6 * u + 0] = (pj.x - xx) * inv_r;
-        f[6 * u + 1] = (pj.y - xy) * inv_r;
-        f[6 * u + 2] = (pj.z - xz) * inv_r;
-        f[6 * u + 3] = (vj.x - vx) * inv_v;
-        f[6 * u + 4] = (vj.y - vy) * inv_v;
-        f[6 * u + 5] = (vj.z - vz) * inv_v;
-      } else {
-#pragma unroll
-        for (int e = 0; e < 6; ++e) f[6 * u + e] = 0.0f;
-      }
+    for (int u = 0; u < kGather; ++u) {
+      const int j = kj[h + u];
+      const int jl = j >= 0 ? j : i;
+      pj[u] = ld3(pos, jl);
+      vj[u] = ld3(vel, jl);
     }
-    // hardware round-to-nearest-even (the defined rounding on every non-NaN
-    // value; a NaN stays a NaN)
+    // two neighbour slots = 12 features = 3 groups of 4
 #pragma unroll
-    for (int gq = 0; gq < 3; ++gq) {
-      const uint2 w = make_uint2(pack_bf16x2(f[4 * gq], f[4 * gq + 1]),
-                                 pack_bf16x2(f[4 * gq + 2], f[4 * gq + 3]));
-      const int grp = 3 * (s / 2) + gq;  // features 4 grp .. 4 grp + 3
-      if (GROUPED) {
-        reinterpret_cast<uint2 *>(feat)[(size_t)grp * n + i] = w;
-      } else {
-        reinterpret_cast<uint2 *>(feat + (size_t)i * kIn)[grp] = w;
+    for (int s = h; s < h + kGather; s += 2) {
+      float f[12];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int j = kj[s + u];
+        if (knn) knn[(size_t)i * kKnn + s + u] = j;
+        const Packed3 &q = pj[s - h + u], &w = vj[s - h + u];
+        const bool has = j >= 0;
+        f[6 * u + 0] = has ? (q.x - xx) * inv_r : 0.0f;
+        f[6 * u + 1] = has ? (q.y - xy) * inv_r : 0.0f;
+        f[6 * u + 2] = has ? (q.z - xz) * inv_r : 0.0f;
+        f[6 * u + 3] = has ? (w.x - vx) * inv_v : 0.0f;
+        f[6 * u + 4] = has ? (w.y - vy) * inv_v : 0.0f;
+        f[6 * u + 5] = has ? (w.z - vz) * inv_v : 0.0f;
+      }
+      // hardware round-to-nearest-even (the defined rounding on every non-NaN
+      // value; a NaN stays a NaN)
+#pragma unroll
+      for (int gq = 0; gq < 3; ++gq) {
+        const uint2 wv = make_uint2(pack_bf16x2(f[4 * gq], f[4 * gq + 1]),
+                                    pack_bf16x2(f[4 * gq + 2], f[4 * gq + 3]));
+        const int grp = 3 * (s / 2) + gq;  // features 4 grp .. 4 grp + 3
+        if (GROUPED) {
+          reinterpret_cast<uint2 *>(feat)[(size_t)grp * n + i] = wv;
+        } else {
+          reinterpret_cast<uint2 *>(feat + (size_t)i * kIn)[grp] = wv;
+        }
       }
     }
   }
