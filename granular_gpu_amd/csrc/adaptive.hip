@@ -60,6 +60,33 @@ __device__ __forceinline__ bool maybe_within(float d2, float lim) {
   return !(lim >= 0x1p-60f) || d2 <= contact_prefilter(lim);
 }
 
+// Calls f(j, x_j, a[j]) for the particles of the stencil in scan order, with
+// the positions and a[j] of four candidates loaded ahead of their calls, so
+// that a candidate's test does not wait for its own loads one at a time.
+template <class T, class F>
+__device__ __forceinline__ void for_stencil_loaded(float px, float py, float pz,
+                                                   const DevGrid &g, const int *__restrict__ cs,
+                                                   const float *__restrict__ pos,
+                                                   const T *__restrict__ a, F &&f) {
+  int bx, by, bz;
+  if (!trunc_cell(px, py, pz, g, bx, by, bz)) return;
+  stencil_rows(bx, by, bz, g, cs, [&](int j0, int j1) {
+    int j = j0;
+    for (; j + 4 <= j1; j += 4) {
+      Packed3 q[4];
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        q[u] = ld3(pos, j + u);
+        v[u] = a[j + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) f(j + u, q[u], v[u]);
+    }
+    for (; j < j1; ++j) f(j, ld3(pos, j), a[j]);
+  });
+}
+
 template <class F>
 __device__ __forceinline__ void for_stencil(float px, float py, float pz, const DevGrid &g,
                                             const int *__restrict__ cs, F &&f) {
@@ -89,11 +116,11 @@ __global__ void __launch_bounds__(kBlock)
   const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
   const float ri = rs[i];
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for_stencil(px, py, pz, g, cs, [&](int j) {
+  for_stencil_loaded(px, py, pz, g, cs, pos, rs, [&](int j, const Packed3 &q, float rj) {
     if (j == i) return;
-    const float bx = px - pos[3 * j], by = py - pos[3 * j + 1], bz = pz - pos[3 * j + 2];
+    const float bx = px - q.x, by = py - q.y, bz = pz - q.z;
     const float d2 = bx * bx + by * by + bz * bz;
-    const float lim = 2.0f * (ri + rs[j]);
+    const float lim = 2.0f * (ri + rj);
     // (a candidate failing the prefilter fails the test: no square root)
     if (!maybe_within(d2, lim)) return;
     const float dis = sqrtf(d2);
@@ -120,11 +147,11 @@ __global__ void __launch_bounds__(kBlock)
   const float3 cv = nrm(c.x, c.y, c.z);
   int nn = 0;
   bool vote = false;
-  for_stencil(px, py, pz, g, cs, [&](int j) {
+  for_stencil_loaded(px, py, pz, g, cs, pos, rs, [&](int j, const Packed3 &q, float rj) {
     if (j == i) return;
-    const float bx = pos[3 * j] - px, by = pos[3 * j + 1] - py, bz = pos[3 * j + 2] - pz;
+    const float bx = q.x - px, by = q.y - py, bz = q.z - pz;
     const float d2 = bx * bx + by * by + bz * bz;
-    const float lim = 2.0f * (ri + rs[j]);
+    const float lim = 2.0f * (ri + rj);
     if (!maybe_within(d2, lim)) return;
     const float dis = sqrtf(d2);
     if (!(dis <= lim)) return;
@@ -180,9 +207,9 @@ __global__ void __launch_bounds__(kBlock)
   }
   const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
   float best = 100000.0f;
-  for_stencil(px, py, pz, g, cs, [&](int j) {
-    if (j == i || surf[j] == 0) return;
-    const float dx = px - pos[3 * j], dy = py - pos[3 * j + 1], dz = pz - pos[3 * j + 2];
+  for_stencil_loaded(px, py, pz, g, cs, pos, surf, [&](int j, const Packed3 &q, int sj) {
+    if (j == i || sj == 0) return;
+    const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
     if (d2 < best) best = d2;
   });
@@ -214,11 +241,13 @@ __global__ void __launch_bounds__(kBlock)
   int ci = -1;
   float ld = 0.0f;
   int lc = 0;
-  for_stencil(px, py, pz, g, cs, [&](int j) {
+  // (a = the radii under the depth rules, else any array of n floats)
+  const float *a = R.depth ? rs : mass;
+  for_stencil_loaded(px, py, pz, g, cs, pos, a, [&](int j, const Packed3 &q, float aj) {
     if (j >= n) return;
-    const float dx = px - pos[3 * j], dy = py - pos[3 * j + 1], dz = pz - pos[3 * j + 2];
+    const float dx = px - q.x, dy = py - q.y, dz = pz - q.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
-    const float lim = partner_dist(R, ri, R.depth ? rs[j] : 0.0f);
+    const float lim = partner_dist(R, ri, R.depth ? aj : 0.0f);
     // everything below needs dis < lim, which a candidate failing the
     // prefilter cannot meet
     if (!maybe_within(d2, lim)) return;
