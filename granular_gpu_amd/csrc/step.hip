@@ -287,7 +287,15 @@ __device__ __forceinline__ void full_stencil(const SweepArgs &a, const SweepCons
                                              float preb, float3 &ab, float3 &ag, int &nb,
                                              int &ng) {
   stencil_rows(bx, by, bz, lg.g, a.bcs + lg.base, [&](int k0, int k1) {
-    for (int k = k0; k < k1; ++k) boundary_contact<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, ab, nb);
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      const float4 q0 = a.bnd4[k], q1 = a.bnd4[k + 1], q2 = a.bnd4[k + 2], q3 = a.bnd4[k + 3];
+      boundary_contact<FRIC>(bi, oi, q0, rcb, preb, sc, ab, nb);
+      boundary_contact<FRIC>(bi, oi, q1, rcb, preb, sc, ab, nb);
+      boundary_contact<FRIC>(bi, oi, q2, rcb, preb, sc, ab, nb);
+      boundary_contact<FRIC>(bi, oi, q3, rcb, preb, sc, ab, nb);
+    }
+    for (; k < k1; ++k) boundary_contact<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, ab, nb);
   });
   stencil_rows(bx, by, bz, lg.g, a.cs + lg.base, [&](int j0, int j1) {
     int j = j0;
@@ -563,9 +571,23 @@ __global__ void __launch_bounds__(BS, 6) k_sweep(SweepArgs a) {
   float3 ab = make_float3(0.0f, 0.0f, 0.0f), ag = make_float3(0.0f, 0.0f, 0.0f);
   int nb = 0, ng = 0;
   if (mode == 1) {
-    for (int t = 0; t < mt.y; ++t) {
-      const int k = a.blist[(size_t)t * sc.cap + i];
-      boundary_contact<FRIC>(bi, oi, a.bnd4[k], rcb, preb, sc, ab, nb);
+    // boundary candidates in list order, gathered four at a time (a particle
+    // beside a dense boundary has dozens)
+    {
+      const int *bl = a.blist + i;
+      const size_t bstride = (size_t)sc.cap;
+      int t = 0;
+      for (; t + 4 <= mt.y; t += 4) {
+        const int k0 = bl[(t + 0) * bstride], k1 = bl[(t + 1) * bstride];
+        const int k2 = bl[(t + 2) * bstride], k3 = bl[(t + 3) * bstride];
+        const float4 q0 = a.bnd4[k0], q1 = a.bnd4[k1], q2 = a.bnd4[k2], q3 = a.bnd4[k3];
+        boundary_contact<FRIC>(bi, oi, q0, rcb, preb, sc, ab, nb);
+        boundary_contact<FRIC>(bi, oi, q1, rcb, preb, sc, ab, nb);
+        boundary_contact<FRIC>(bi, oi, q2, rcb, preb, sc, ab, nb);
+        boundary_contact<FRIC>(bi, oi, q3, rcb, preb, sc, ab, nb);
+      }
+      for (; t < mt.y; ++t)
+        boundary_contact<FRIC>(bi, oi, a.bnd4[bl[t * bstride]], rcb, preb, sc, ab, nb);
     }
     // candidates in list order, gathered four at a time
     const int *gl = a.glist + i;
