@@ -296,12 +296,14 @@ def scale_phase(args, lib, torch, C, np, learned):
     learned.infer(sim, fine_pos=lpos)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    # 100 calls (~25 ms): a window long enough for a stable average
+    calls = 100
     e0.record()
-    for _ in range(steps):
+    for _ in range(calls):
         learned.infer(sim, fine_pos=lpos)
     e1.record()
     torch.cuda.synchronize()
-    t_all = 1000.0 * e0.elapsed_time(e1) / steps
+    t_all = 1000.0 * e0.elapsed_time(e1) / calls
     _, tfp, _ = learned.teacher(sim)
     err = fine_error(lpos, tfp) / 0.01
     print(f"# scale: plain {plain_us:.1f} us, learned {t_all:.1f} us, error {err:.4f} r_fine",
