@@ -192,7 +192,7 @@ _PROTOS = {
     "gg_upsample_mlp_scratch_bytes": (C.c_size_t, [C.c_int]),
     "gg_upsample_mlp": (C.c_int, [P(gg_particles), VP, P(gg_particles), VP, gg_grid,
                                   P(gg_upsample_params), P(gg_mlp_config), VP, VP, VP, VP, VP,
-                                  C.c_size_t, P(C.c_int), VP]),
+                                  C.c_size_t, VP, VP]),
     "gg_mfma_probe": (C.c_int, [VP, VP, VP, VP]),
     "gg_bf16_round_check": (C.c_int, [C.c_uint64, C.c_uint64, VP, VP]),
     "gg_fast_math_check": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, VP, VP]),

@@ -452,6 +452,7 @@ __global__ void __launch_bounds__(256)
 }
 
 __global__ void k_zero_int(int *p) { *p = 0; }
+__global__ void k_copy_int(const int *src, int *dst) { *dst = *src; }
 
 __global__ void __launch_bounds__(256)
     k_mlp_teacher(int n, const float *__restrict__ pos, const float *__restrict__ vel,
@@ -1213,6 +1214,7 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   k_mlp_flags<<<nblocks(n, 256), 256, 0, side->st>>>(
       n, pos, reinterpret_cast<const float *>(boundary->pos), bcs, g, mlp_const(c), flags,
       fb_list, fb_count);
+  if (fallback_count) k_copy_int<<<1, 1, 0, side->st>>>(fb_count, fallback_count);
   const bool flags_ok =
       hipGetLastError() == hipSuccess && hipEventRecord(side->flags, side->st) == hipSuccess;
   if (flags_ok)
@@ -1261,7 +1263,6 @@ extern "C" int gg_upsample_mlp(const gg_particles *coarse, const int *cs,
   }
   prof_end(GG_STAGE_MLP, stream);
   if (hipStreamWaitEvent(stream, side->join, 0) != hipSuccess) return GG_ERR_HIP;
-  (void)fallback_count;
   return GG_OK;
 }
 

@@ -168,7 +168,7 @@ class LearnedUpsampler:
             _stream()))
         return y, fp, fv
 
-    def infer(self, sim, fine_pos=None, fine_vel=None):
+    def infer(self, sim, fine_pos=None, fine_vel=None, fallback_count=None):
         n = sim.n
         dev = sim.pos.device
         need = int(self.lib.gg_upsample_mlp_scratch_bytes(n))
@@ -179,8 +179,8 @@ class LearnedUpsampler:
         check("gg_upsample_mlp", self.lib.gg_upsample_mlp(
             C.byref(sim.set), _ptr(sim.cs), C.byref(sim.bset), _ptr(sim.bcs), sim.grid,
             C.byref(self.up), C.byref(self.cfg), _ptr(self.net.w_bf16), _ptr(self.net.bias),
-            _ptr(fine_pos), _ptr(fine_vel), _ptr(self._scratch), self._scratch.numel(), None,
-            _stream()))
+            _ptr(fine_pos), _ptr(fine_vel), _ptr(self._scratch), self._scratch.numel(),
+            _ptr(fallback_count), _stream()))
         return fine_pos
 
     def rigid(self, sim):

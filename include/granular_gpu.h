@@ -454,7 +454,9 @@ int gg_mlp_adam(gg_mlp_weights *w, const float *dW, const float *db, float lr, f
 int gg_mlp_sync_weights(gg_mlp_weights *w, hipStream_t stream);
 /* Inference: features, fused register-resident MFMA network, teacher on
  * the flagged particles.  fine_pos / fine_vel: n x 24 (8 fine per coarse,
- * fine_vel optional); scratch: gg_upsample_mlp_scratch_bytes(n).  The teacher
+ * fine_vel optional); scratch: gg_upsample_mlp_scratch_bytes(n);
+ * fallback_count (optional, one int of device memory) receives the number of
+ * flagged particles, whose outputs the teacher wrote.  The teacher
  * runs on a library-owned second stream (one per device, default priority)
  * forked from and joined back into `stream` by events, so the call stays
  * ordered on `stream`; calls that overlap on one device share that stream. */

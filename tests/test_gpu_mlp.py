@@ -155,7 +155,8 @@ def test_inference_matches_training_forward_and_fallback(gpu):
     s, sim = _pile()
     net = MlpNet(seed=5)
     up = LearnedUpsampler(net)
-    fine = up.infer(sim)
+    flagged = torch.full((1,), -1, dtype=torch.int32, device="cuda")
+    fine = up.infer(sim, fallback_count=flagged)
     feat, flags, _ = up.features(sim)
     tr = Trainer(net, batch=sim.n)
     y = tr.forward(feat).cpu().numpy()
@@ -165,6 +166,7 @@ def test_inference_matches_training_forward_and_fallback(gpu):
     _, tfp, _ = up.teacher(sim)
     fl = flags.cpu().numpy()
     got = fine.cpu().numpy()
+    assert int(flagged.item()) == int((fl != 0).sum()) > 0
     assert np.array_equal(got[fl != 0], tfp.cpu().numpy()[fl != 0])
     d = np.abs(got[fl == 0] - want[fl == 0]).max()
     _log(f"inference vs training forward: max |dx| {d:.2e} m over {int((fl == 0).sum())} coarse")
