@@ -322,6 +322,65 @@ def scale_phase(args, lib, torch, C, np, learned):
     }
 
 
+def excavator_phase(args, lib, torch, C, np):
+    """The excavator dig at size through the host system: 4M particles of
+    r = 0.004, a 200k-particle bucket on the scripted arc, adaptive merge/split
+    with the radius-level rules, 160-candidate lists with the mid-step
+    refresh, and the learned upsampler writing 8 fine particles per granular
+    particle from GranularSystem::step (an untrained network: the weights do
+    not change the coarse run or the inference cost).  Per-stage times from
+    the library's stage profiler over steps --excavator-from..--excavator-to."""
+    import math
+    from granular_gpu_amd import _lib, host, scenes
+    from granular_gpu_amd.learned import init_master
+
+    r = 0.004
+    a, b = args.excavator_from, args.excavator_to
+    dig = scenes.excavator_dig(bed=(2.0, 1.0), fill=1.11, r=r, bucket_count=200_000)
+    s = dig.scene
+    s.density = np.float32(3.0 / (4.0 * math.pi * r ** 3))
+    sysm = host.HostSystem.from_scene(s, is_move_boundary=True, is_adaptive=1,
+                                      boundary_animated=dig.animated)
+    sysm.set_merge_rules()
+    sysm.set_lists(gmax=160, bmax=96, skin=0.75 * r)
+    sysm.set_solver(3, 5, 0.0, 0.0, _lib.GG_FLAG_LIST_REFRESH)
+    W, bias = init_master(6)
+    sysm.set_learned((W.view(np.uint32) >> 16).astype(np.uint16), bias, radius=r,
+                     guard_boundary=2.5 * r, guard_floor=4 * r, guard_side=2.5 * r)
+    st = host.animation_default()
+    for key, v in dig.script.items():
+        if key in ("translation_direction", "rotation_center"):
+            getattr(st, key)[:] = tuple(float(x) for x in v)
+        else:
+            setattr(st, key, v)
+    dt = float(s.dt)
+    print(f"# excavator: n={s.n} boundary={s.boundary.shape[0]}, steps {a}..{b}",
+          file=sys.stderr, flush=True)
+    for k in range(b):
+        if k == a:
+            lib.gg_profiler_enable(200000)
+        sysm.update_animation(st, dt)
+        sysm.step(1)
+        if (k + 1) % 100 == 0:
+            print(f"# excavator step {k + 1}: {sysm.size('granular')} particles",
+                  file=sys.stderr, flush=True)
+    tot = (C.c_double * len(_lib.STAGES))()
+    cnt = (C.c_int * len(_lib.STAGES))()
+    _lib.check("gg_profiler_collect", lib.gg_profiler_collect(tot, cnt, len(_lib.STAGES)))
+    lib.gg_profiler_enable(0)
+    ft = sysm.frame_times()
+    out = {"workload": f"excavator dig, {s.n} particles r = {r} at step 0, 200k-particle bucket, "
+                       f"adaptive (radius levels), learned 8-per-particle output, steps {a}..{b}",
+           "particles": int(sysm.size("granular")), "fine": int(sysm.size("fine")),
+           "frame_ms": round(float(np.mean(ft[a:b])), 3),
+           "stages_ms_per_step": {name: round(tot[k] / (b - a), 3)
+                                  for k, name in enumerate(_lib.STAGES) if cnt[k]},
+           "launches_per_step": {name: round(cnt[k] / (b - a), 2)
+                                 for k, name in enumerate(_lib.STAGES) if cnt[k]}}
+    sysm.close()
+    return out
+
+
 def batch_phase(args, lib, torch, C, np):
     """Batched scenes: 64 independent plain box scenes with particle counts
     log-uniform in [4k, 256k] (seeded; 4.17M particles in all), per-scene
@@ -443,8 +502,11 @@ def main():
     ap.add_argument("--upsample-box", type=float, default=3.0)
     ap.add_argument("--upsample-steps", type=int, default=100)
     ap.add_argument("--upsample-warmup", type=int, default=20)
-    ap.add_argument("--only", choices=["upsample", "batch"], default=None,
-                    help="run one secondary phase alone and print its object (profiling)")
+    ap.add_argument("--only", choices=["upsample", "batch", "excavator"], default=None,
+                    help="run one secondary phase alone and print its object (profiling; "
+                         "the excavator runs only this way)")
+    ap.add_argument("--excavator-from", type=int, default=500)
+    ap.add_argument("--excavator-to", type=int, default=600)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--scale-side", type=int, default=100,
                     help="coarse lattice side of the 8M-fine comparison (0 skips it)")
@@ -497,9 +559,10 @@ def main():
         print(json.dumps({"error": "device is not gfx950"}), flush=True)
         sys.exit(1)
 
-    if args.only in ("upsample", "batch"):
+    if args.only in ("upsample", "batch", "excavator"):
         # one secondary phase alone (for profiling it in isolation)
-        fn = upsample_phase if args.only == "upsample" else batch_phase
+        fn = {"upsample": upsample_phase, "batch": batch_phase,
+              "excavator": excavator_phase}[args.only]
         print(json.dumps({"only": args.only, args.only: fn(args, lib, torch, C, np)}),
               flush=True)
         return
