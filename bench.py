@@ -381,6 +381,53 @@ def excavator_phase(args, lib, torch, C, np):
     return out
 
 
+def funnel_phase(args, lib, torch, C, np):
+    """The funnel pour at size: 1M particles dropped as a column onto a cone
+    of 400k boundary particles, adaptive merge/split with the radius-level
+    rules (masses 0.125 / 1 / 8; merge deeper than 3 r below the free surface,
+    split within 2 r of it), 3 + 5 sweeps.  Steps 0..400 untimed (the column
+    lands and starts draining at step ~360), then steps 400..500 timed, with
+    the library's stage profiler on.  Lists of 160 granular / 96 boundary
+    candidates (a particle on the cone has more than 16 boundary candidates:
+    with 16 it scans its stencil every sweep, 15.9 against 10.5 ms per step) and the
+    mid-step list refresh; the lists do not change the results."""
+    from granular_gpu_amd import _lib, scenes
+    from granular_gpu_amd.device import AdaptiveSim
+
+    s = scenes.funnel_column()
+    sim = AdaptiveSim(s, capacity=4 * s.n, max_mass=8.0, stab=3, solver=5, gmax=160, bmax=96,
+                      flags=_lib.GG_FLAG_LIST_REFRESH)
+    sim.params.adaptive_rules = 1
+    for key, v in dict(min_mass=0.125, split_mass=0.125, merge_dist=1.75, merge_depth=3.0,
+                       split_depth=2.0).items():
+        setattr(sim.params, key, v)
+    print(f"# funnel: n={s.n} boundary={s.boundary.shape[0]}", file=sys.stderr, flush=True)
+    a, b = 400, 500
+    for _ in range(a // 100):
+        sim.step(100)
+        torch.cuda.synchronize()
+        print(f"# funnel: {sim.n} particles", file=sys.stderr, flush=True)
+    n0 = int(sim.n)
+    lib.gg_profiler_enable(200000)
+    t0 = time.perf_counter()
+    sim.step(b - a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    tot = (C.c_double * len(_lib.STAGES))()
+    cnt = (C.c_int * len(_lib.STAGES))()
+    _lib.check("gg_profiler_collect", lib.gg_profiler_collect(tot, cnt, len(_lib.STAGES)))
+    lib.gg_profiler_enable(0)
+    stats = np.array(sim.stats, np.int64)[a:b]
+    psteps = int(stats[:, 2].sum())  # particles at the start of each timed step
+    return {"workload": f"funnel column, {s.n} particles over a {s.boundary.shape[0]}-particle "
+                        f"cone and walls, radius-level merge/split, steps {a}..{b}",
+            "particles": [n0, int(sim.n)], "ms_per_step": round(1000 * el / (b - a), 3),
+            "particle_steps_per_s": round(psteps / el, 1),
+            "merges": int(stats[:, 0].sum()), "splits": int(stats[:, 1].sum()),
+            "stages_ms_per_step": {name: round(tot[k] / (b - a), 3)
+                                   for k, name in enumerate(_lib.STAGES) if cnt[k]}}
+
+
 def batch_phase(args, lib, torch, C, np):
     """Batched scenes: 64 independent plain box scenes with particle counts
     log-uniform in [4k, 256k] (seeded; 4.17M particles in all), per-scene
@@ -502,9 +549,10 @@ def main():
     ap.add_argument("--upsample-box", type=float, default=3.0)
     ap.add_argument("--upsample-steps", type=int, default=100)
     ap.add_argument("--upsample-warmup", type=int, default=20)
-    ap.add_argument("--only", choices=["upsample", "batch", "excavator"], default=None,
+    ap.add_argument("--only", choices=["upsample", "batch", "excavator", "funnel"],
+                    default=None,
                     help="run one secondary phase alone and print its object (profiling; "
-                         "the excavator runs only this way)")
+                         "the excavator and the funnel run only this way)")
     ap.add_argument("--excavator-from", type=int, default=500)
     ap.add_argument("--excavator-to", type=int, default=600)
     ap.add_argument("--no-learned", action="store_true")
@@ -559,10 +607,10 @@ def main():
         print(json.dumps({"error": "device is not gfx950"}), flush=True)
         sys.exit(1)
 
-    if args.only in ("upsample", "batch", "excavator"):
+    if args.only in ("upsample", "batch", "excavator", "funnel"):
         # one secondary phase alone (for profiling it in isolation)
         fn = {"upsample": upsample_phase, "batch": batch_phase,
-              "excavator": excavator_phase}[args.only]
+              "excavator": excavator_phase, "funnel": funnel_phase}[args.only]
         print(json.dumps({"only": args.only, args.only: fn(args, lib, torch, C, np)}),
               flush=True)
         return
