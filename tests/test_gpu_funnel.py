@@ -26,10 +26,10 @@ def _log(msg):
     sys.stdout.flush()
 
 
-def _sim(scene, cap_factor):
+def _sim(scene, cap_factor, **lists):
     from granular_gpu_amd.device import AdaptiveSim
     sim = AdaptiveSim(scene, capacity=cap_factor * scene.n, max_mass=MAX_MASS, stab=3,
-                      solver=5)
+                      solver=5, **lists)
     sim.params.adaptive_rules = 1
     for k, v in DEPTH.items():
         setattr(sim.params, k, v)
@@ -102,3 +102,28 @@ def test_funnel_1m_cone_400k_depth_rules_properties(gpu):
     # the column drains through the outlet: nothing below it before the fall
     # time of the axis particles (2.52 m, 0.72 s, step 359), then more and more
     assert below[0] == 0 and below[-1] > below[-2] > 0
+
+
+def test_funnel_1m_lists_do_not_change_results(gpu):
+    # 16 boundary slots overflow beside the cone (those particles scan their
+    # stencils); 96 slots, 160 granular ones and the mid-step refresh use the
+    # lists there instead: the same state bit for bit after 300 steps
+    from granular_gpu_amd._lib import GG_FLAG_LIST_REFRESH
+    s = scenes.funnel_column()
+    out = []
+    for lists in (dict(), dict(gmax=160, bmax=96, flags=GG_FLAG_LIST_REFRESH)):
+        sim = _sim(s, 4, **lists)
+        sim.step(300)
+        torch.cuda.synchronize()
+        n = sim.n
+        out.append((n, sim.pos[:n].cpu().numpy(), sim.vel[:n].cpu().numpy(),
+                    sim.mass[:n].cpu().numpy(), np.array(sim.stats, np.int64)))
+        _log(f"lists {lists}: n {n}, merged {out[-1][4][:, 0].sum()}, "
+             f"split {out[-1][4][:, 1].sum()}")
+        del sim
+        torch.cuda.empty_cache()
+    (na, *a), (nb, *b) = out
+    assert na == nb
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert a[3][:, 0].sum() > 0 and a[3][:, 1].sum() > 0
